@@ -1,0 +1,224 @@
+/* pcf.h -- C-ABI of libpcf, the MI355X (gfx950) Monte Carlo / binomial
+ * pricing engine.
+ *
+ * Plain C types only: no HIP, no torch, no C++ across this boundary.  Every
+ * entry point returns an int status (PCF_OK = 0); no exception crosses it.
+ * Caller owns all parameter and result memory; a pcf_ctx owns every device
+ * buffer (resident between calls) and one HIP stream.  A context is used by
+ * one host thread at a time (calls on one context are serialised internally).
+ *
+ * Each pricing entry point replaces one reference kernel function and keeps
+ * its argument list and meaning (S0, E, r, sigma, T, N[, M][, assets, rho],
+ * payoff_fun = +1 call / -1 put), then adds options and a result record.
+ */
+#ifndef PCF_H
+#define PCF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCF_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum pcf_status {
+    PCF_OK = 0,
+    PCF_ERR_INVALID_ARG = 1,   /* bad payoff / size / parameter (the reference throws
+                                  std::invalid_argument: src/mc_eur.cpp:42,
+                                  include/common.h:180) */
+    PCF_ERR_NO_DEVICE = 2,     /* no HIP device / device index out of range */
+    PCF_ERR_HIP = 3,           /* a HIP runtime call failed (message in last_error) */
+    PCF_ERR_OUT_OF_MEMORY = 4, /* device allocation failed, or a request exceeds the
+                                  server's memory budget */
+    PCF_ERR_SINGULAR = 5,      /* reference-exact LSM: Gram determinant <= 0
+                                  (include/common.h:115-117 throws) */
+    PCF_ERR_NOT_FOUND = 6,     /* unknown server ticket */
+    PCF_ERR_SHUTDOWN = 7,      /* server is shutting down */
+    PCF_ERR_TIMEOUT = 8        /* a bounded device wait expired */
+};
+
+/* ---- arithmetic form ---------------------------------------------------- */
+enum pcf_precision {
+    PCF_FP32K = 0, /* FP32 path evolution and payoffs; per-thread and per-block payoff
+                      sums Kahan-compensated in FP64; FP64 final mean/variance */
+    PCF_FP64 = 1   /* everything in FP64 */
+};
+
+/* ---- normal source ------------------------------------------------------ */
+enum pcf_rng {
+    PCF_RNG_PHILOX = 0, /* counter-based Philox4x32-10 + Box-Muller, keyed by
+                           (seed, path, step): key = (seed lo, seed hi),
+                           counter = (step/4, path lo, path hi, stream id);
+                           uniform u = (x + 0.5) 2^-32; the pair (u0,u1) gives
+                           R cos(2 pi u1), R sin(2 pi u1), R = sqrt(-2 ln u0).
+                           Any path is reproducible regardless of launch shape. */
+    PCF_RNG_HOST = 1    /* verification: the caller passes FP64 normals in the
+                           reference's draw order (see each entry point); the
+                           device replays the reference arithmetic in FP64 */
+};
+
+/* Philox stream ids (counter word 3). */
+enum pcf_stream {
+    PCF_STREAM_EUR = 1,    /* normal p%4 of block 0 of Philox path p/4 */
+    PCF_STREAM_ASIA = 2,   /* step m of path p: normal m of path p */
+    PCF_STREAM_AMER = 3,   /* date m (1..M) of antithetic pair n: normal m-1 of path n */
+    PCF_STREAM_BASKET = 4  /* asset a of path p: normal a of path p */
+};
+
+typedef struct pcf_opts {
+    uint64_t seed;           /* Philox key */
+    int32_t precision;       /* enum pcf_precision */
+    int32_t rng;             /* enum pcf_rng */
+    const double* host_normals; /* PCF_RNG_HOST only */
+    int64_t n_host_normals;
+    int32_t ref_exact;       /* 1: replay the reference's arithmetic choices exactly
+                                (binom: comb() chain; mc_amer: basis {1,S-E,(S-E)^2},
+                                cofactor inverse and the payoff(S-E,E) exercise test;
+                                basket: no sqrt(T) on the correlated draws) */
+    int32_t control;         /* mc_asia: also price the geometric-average control */
+    int32_t block_size;      /* threads per block; 0 = tuned default */
+    int32_t items_per_thread;/* paths (or path groups) per thread; 0 = tuned default */
+    int32_t* when_out;       /* mc_amer: optional host array of N exercise dates */
+    int32_t reserved[6];
+} pcf_opts;
+
+typedef struct pcf_result {
+    double price;        /* discounted price: the reference function's return value */
+    double std_err;      /* standard error of price (0 for the binomial sum) */
+    double mean;         /* FP64 mean of undiscounted path payoffs (binom: term sum) */
+    double variance;     /* FP64 sample variance of undiscounted path payoffs */
+    double control_price;/* mc_asia: geometric-average control price on the same paths;
+                            mc_amer: European price on the same paths (no exercise) */
+    double control_se;
+    int64_t n_paths;
+    int64_t n_steps;     /* time steps per path (1 for single-step products) */
+    double kernel_ms;    /* device time of the dominant kernel, HIP events on the
+                            launch stream */
+    double total_ms;     /* device time of the whole launch sequence */
+    double flops;        /* algorithmic flop count of the dominant kernel */
+    double bytes;        /* algorithmic HBM bytes of the dominant kernel */
+    int32_t lsm_small_itm; /* mc_amer: dates with 1..2 in-the-money paths */
+    int32_t lsm_fallback;  /* mc_amer: dates whose Gram solve fell back to a smaller basis */
+    int32_t grid_blocks;   /* launch shape of the dominant kernel */
+    int32_t block_size;
+    int32_t reserved[4];
+} pcf_result;
+
+typedef struct pcf_ctx pcf_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int pcf_ctx_create(int device, pcf_ctx** out);
+int pcf_ctx_destroy(pcf_ctx* ctx);
+const char* pcf_ctx_last_error(const pcf_ctx* ctx);
+int pcf_ctx_synchronize(pcf_ctx* ctx);
+/* Device name, compute units, device memory in bytes. */
+int pcf_device_info(pcf_ctx* ctx, char* name, int name_len, int* compute_units,
+                    int64_t* total_mem);
+void pcf_opts_init(pcf_opts* opts);
+int pcf_abi_version(void);
+/* sizeof of an ABI record by name ("pcf_opts", "pcf_result", "pcf_basket_ext",
+ * "pcf_request", "pcf_server_config", "pcf_request_timing"); -1 if unknown.
+ * Lets FFI bindings verify their struct mirrors without a device. */
+int pcf_struct_size(const char* name);
+
+/* ---- pricing entry points ----------------------------------------------- */
+
+/* European option on one GBM asset, single terminal step.
+ * Replaces double mc_eur(S0,E,r,sigma,T,N,payoff_fun), src/mc_eur.cpp:5-27.
+ * PCF_RNG_HOST: host_normals[n], n < N, are the draws N(0, sqrt(T)). */
+int pcf_mc_eur(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+               int64_t N, double payoff_fun, const pcf_opts* opts, pcf_result* res);
+
+/* Closed binomial sum over the beta tree (src/binom_embar.cpp:19-27).
+ * Replaces double binom(S0,E,r,sigma,T,N,payoff_fun), src/binom_embar.cpp:5-50.
+ * ref_exact = 1 (default through the CLI) replays comb() of include/common.h:63-72
+ * bit for bit; ref_exact = 0 uses FP64 lgamma coefficients. */
+int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                    int64_t N, double payoff_fun, const pcf_opts* opts, pcf_result* res);
+
+/* Fixed-strike arithmetic Asian with the reference's trapezoid running sum.
+ * Replaces double mc_asia(S0,E,r,sigma,T,N,M,payoff_fun), src/mc_asia.cpp:5-40.
+ * PCF_RNG_HOST: host_normals[n*M+m] are the draws N(0, sqrt(T/M)). */
+int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
+                pcf_result* res);
+
+/* American option by Longstaff-Schwartz on an antithetic path store.
+ * Replaces double mc_amer(S0,E,r,sigma,T,N,M,payoff_fun), src/mc_amer.cpp:5-114
+ * (paths: include/common.h:168-208).  N must be even.
+ * PCF_RNG_HOST: host_normals[n*M+(m-1)], n < N/2, m = 1..M, are the draws
+ * N(0, sqrt(T/M)) in pathsfinder's order. */
+int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
+                pcf_result* res);
+
+/* Optional per-asset extension of the basket (NULL pointers = the reference's
+ * scalar sigma, S0, weights 1/d and equicorrelation rho). */
+typedef struct pcf_basket_ext {
+    const double* sigma;   /* d volatilities */
+    const double* S0;      /* d spot prices */
+    const double* weights; /* d weights */
+    const double* corr;    /* d x d correlation matrix, row-major, SPD */
+} pcf_basket_ext;
+
+/* Arithmetic basket call/put on d correlated GBM assets, single terminal step.
+ * Replaces double mc_eur(S0,E,r,sigma,T,N,payoff_fun,assets,rho),
+ * src/mc_eur_multi.cpp:6-35, with mvnorm of include/mvn.h:42-82.
+ * PCF_RNG_HOST: host_normals[n*d+a] are standard normals (column-major d x N,
+ * the order of mvnorm's NullaryExpr); the device applies the Cholesky factor. */
+int pcf_mc_eur_multi(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                     int64_t N, double payoff_fun, int32_t assets, double rho,
+                     const pcf_basket_ext* ext, const pcf_opts* opts, pcf_result* res);
+
+/* Standard normals of the device generator: out[p*K + k] = normal k of path
+ * path0+p of `stream` (the exact values the pricing kernels consume, in FP64
+ * or rounded to FP32 per `precision`).  For reproducibility checks. */
+int pcf_normals(pcf_ctx* ctx, uint64_t seed, uint32_t stream, uint64_t path0,
+                int64_t n_paths, int32_t K, int32_t precision, double* out);
+
+/* ---- serving ------------------------------------------------------------ */
+enum pcf_kind { PCF_KIND_EUR = 0, PCF_KIND_ASIA = 1, PCF_KIND_AMER = 2,
+                PCF_KIND_BASKET = 3, PCF_KIND_BINOM = 4 };
+
+typedef struct pcf_request {
+    int32_t kind;        /* enum pcf_kind */
+    int32_t M;           /* steps (asia) / exercise dates (amer) */
+    int32_t assets;      /* basket dimension */
+    int32_t pad;
+    double S0, E, r, sigma, T, payoff_fun, rho;
+    int64_t N;
+    pcf_opts opts;       /* seed, precision; rng must be PCF_RNG_PHILOX */
+} pcf_request;
+
+typedef struct pcf_server_config {
+    int64_t mem_budget_bytes; /* device bytes the server may hold for requests; 0 = 8 GiB */
+    int32_t max_inflight;     /* concurrent requests on the device; 0 = 8 */
+    int32_t use_graphs;       /* 1: replay captured hipGraphs per (kind, size class) */
+    int32_t reserved[8];
+} pcf_server_config;
+
+typedef struct pcf_request_timing {
+    double submit_ms;    /* host clock at submit, ms since server start */
+    double start_ms;     /* admitted and launched */
+    double done_ms;      /* device work complete */
+    double latency_ms;   /* done - submit */
+} pcf_request_timing;
+
+typedef struct pcf_server pcf_server;
+
+int pcf_server_create(pcf_ctx* ctx, const pcf_server_config* cfg, pcf_server** out);
+/* Queue a request; returns at once with a ticket.  PCF_ERR_OUT_OF_MEMORY if the
+ * request alone can never fit the budget. */
+int pcf_server_submit(pcf_server* srv, const pcf_request* req, uint64_t* ticket);
+/* Block until the ticket's request completes; the ticket is then released. */
+int pcf_server_wait(pcf_server* srv, uint64_t ticket, pcf_result* res,
+                    pcf_request_timing* timing);
+int pcf_server_destroy(pcf_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCF_H */

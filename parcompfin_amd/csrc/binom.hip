@@ -1,0 +1,215 @@
+// binom.hip -- closed binomial sum over the beta tree (src/binom_embar.cpp:5-50).
+//
+//   price = e^{-rT} sum_i C(N,i) p^i q^{N-i} payoff(S0 u^i d^{N-i})
+//
+// with the reference's mirrored pairing: term index i < ceil(N/2) contributes
+// the terms i and N-i sharing one log C(N,i), and for even N the middle term
+// i = N/2 once.  Here lane i (0 <= i < ceil(N/2), plus lane N/2 for even N)
+// owns term i and, if i < ceil(N/2), its mirror N-i; the middle term's
+// expression comb(N,N/2) + N/2 log p + N/2 log q is exactly term 1 of lane N/2.
+//
+// Two coefficient forms:
+//  * chain (reference-exact): log C(N,i) replayed bit for bit as the
+//    reference's sequential chain (include/common.h:63-72):
+//        acc = sum_{n=N..i+1} log n  (descending),  then acc -= log j, j=2..N-i
+//    with glibc log values from a host table.  The table index is uniform
+//    across a wavefront, so loads are scalar broadcasts; lanes that have
+//    reached their own end are masked, never reordered.
+//  * lgamma (production): lgamma(N+1) - lgamma(i+1) - lgamma(N-i+1) in FP64.
+#include <cmath>
+#include <vector>
+
+#include "pcf_internal.hpp"
+
+namespace {
+
+struct BinomArgs {
+    double S0, E, pf, u, d, logp, logq;
+    int64_t N;
+    int64_t lanes;   // number of term lanes
+    int64_t until;   // ceil(N/2): lanes below this also own the mirror term
+};
+
+__device__ __forceinline__ double term_value(const BinomArgs& a, double comb_val,
+                                             int64_t k /*exponent of u*/) {
+#pragma clang fp contract(off)
+    // binom = comb_val + k*log(p) + (N-k)*log(q)  (src/binom_embar.cpp:36-37)
+    const double b = comb_val + (double)k * a.logp + (double)(a.N - k) * a.logq;
+    const double S = a.S0 * pow(a.u, (double)k) * pow(a.d, (double)(a.N - k));
+    const double v = a.pf * (S - a.E);
+    return exp(b) * (v > 0.0 ? v : 0.0);
+}
+
+// Chain form.  One lane per term index; all 64 lanes of a wave run the same
+// trip counts (wave-uniform loop bounds from the wave's first index).
+__global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* __restrict__ tab,
+                                                     pcf::CSum* __restrict__ partials,
+                                                     unsigned int* __restrict__ counter,
+                                                     double* __restrict__ out) {
+    __shared__ pcf::CSum lds[4];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // First index of this wave (uniform).
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(pcf::kWave - 1));
+    const int N = (int)a.N;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)i0);
+    double acc = 0.0;
+    // Part 1: n = N down to i+1.  Unmasked while n > i0+63 (every lane has i <= i0+63).
+    {
+        int n = N;
+        const int lo = w0 + pcf::kWave;  // unmasked for n >= lo
+        for (; n - 7 >= lo; n -= 8) {
+            acc += tab[n];
+            acc += tab[n - 1];
+            acc += tab[n - 2];
+            acc += tab[n - 3];
+            acc += tab[n - 4];
+            acc += tab[n - 5];
+            acc += tab[n - 6];
+            acc += tab[n - 7];
+        }
+        for (; n >= lo; --n) acc += tab[n];
+        for (; n > w0; --n) {
+            const double t = tab[n];
+            acc = (n > i) ? acc + t : acc;
+        }
+    }
+    // Part 2: j = 2 .. N-i.  Unmasked while j <= N-(i0+63).
+    {
+        int j = 2;
+        const int hi_all = N - (w0 + pcf::kWave - 1);
+        for (; j + 7 <= hi_all; j += 8) {
+            acc -= tab[j];
+            acc -= tab[j + 1];
+            acc -= tab[j + 2];
+            acc -= tab[j + 3];
+            acc -= tab[j + 4];
+            acc -= tab[j + 5];
+            acc -= tab[j + 6];
+            acc -= tab[j + 7];
+        }
+        for (; j <= hi_all; ++j) acc -= tab[j];
+        const int hi_any = N - w0;
+        for (; j <= hi_any; ++j) {
+            const double t = tab[j];
+            acc = (j <= N - i) ? acc - t : acc;
+        }
+    }
+    double comb_val = acc;
+    if (i == 0) comb_val = 0.0;                // i==0 || i==N
+    else if (i == 1) comb_val = tab[N];        // i==1 || i==N-1: log(N)
+    const double s = (i < a.lanes) ? term_value(a, comb_val, i) : 0.0;
+    const double s_b = (i < a.until) ? term_value(a, comb_val, a.N - i) : 0.0;
+    pcf::Acc<1> ac, tot;
+    ac.v[0] = {s, 0.0};
+    pcf::csum_add(ac.v[0], s_b);
+    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot) && threadIdx.x == 0) {
+        out[0] = tot.v[0].s + tot.v[0].c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_binom_lgamma(BinomArgs a, pcf::CSum* __restrict__ partials,
+                                                      unsigned int* __restrict__ counter,
+                                                      double* __restrict__ out) {
+    __shared__ pcf::CSum lds[4];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0, s_b = 0.0;
+    if (i < a.lanes) {
+        const double lgN = lgamma((double)a.N + 1.0);
+        const double comb_val = lgN - lgamma((double)i + 1.0) - lgamma((double)(a.N - i) + 1.0);
+        s = term_value(a, comb_val, i);
+        if (i < a.until) s_b = term_value(a, comb_val, a.N - i);
+    }
+    pcf::Acc<1> ac, tot;
+    ac.v[0] = {s, 0.0};
+    pcf::csum_add(ac.v[0], s_b);
+    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot) && threadIdx.x == 0)
+        out[0] = tot.v[0].s + tot.v[0].c;
+}
+
+// glibc log(n) for n = 0..N (entry 0 unused), resident per N.
+int ensure_logtab(pcf_ctx* ctx, int64_t N) {
+    if (ctx->logtab_n == N) return PCF_OK;
+    std::vector<double> t((size_t)N + 1);
+    t[0] = 0.0;
+    for (int64_t n = 1; n <= N; ++n) t[(size_t)n] = std::log((double)n);
+    PCF_HIP_CHECK(ctx, ctx->logtab.ensure(t.size() * sizeof(double)));
+    PCF_HIP_CHECK(ctx, hipMemcpy(ctx->logtab.ptr, t.data(), t.size() * sizeof(double),
+                                 hipMemcpyHostToDevice));
+    ctx->logtab_n = N;
+    return PCF_OK;
+}
+
+}  // namespace
+
+extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, double sigma,
+                               double T, int64_t N, double payoff_fun, const pcf_opts* opts_in,
+                               pcf_result* res) {
+    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    pcf_opts o = pcf_opts_or_default(opts_in);
+    int st = pcf_check_common(ctx, S0, E, r, sigma, T, N, payoff_fun);
+    if (st) return st;
+    if ((st = pcf_check_opts(ctx, &o))) return st;
+    if (N > (int64_t)2147483647 - 64)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "N exceeds the reference's int range");
+    PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    *res = pcf_result{};
+
+    // Tree parameters exactly as src/binom_embar.cpp:16-24 (host, glibc).
+    const double dt = (double)T / (double)N;
+    const double beta = 0.5 * (std::exp(-r * dt) + std::exp((r + std::pow(sigma, 2)) * dt));
+    const double u = beta + std::sqrt(std::pow(beta, 2) - 1);
+    const double d = beta - std::sqrt(std::pow(beta, 2) - 1);
+    const double R = std::exp(r * dt);
+    const double p = (R - d) / (u - d);
+    const double q = 1 - p;
+    BinomArgs a{};
+    a.S0 = S0;
+    a.E = E;
+    a.pf = payoff_fun;
+    a.u = u;
+    a.d = d;
+    a.logp = std::log(p);
+    a.logq = std::log(q);
+    a.N = N;
+    a.until = (N % 2 != 0) ? (N + 1) / 2 : N / 2;
+    a.lanes = a.until + ((N % 2 == 0) ? 1 : 0);
+
+    const int B = 256;
+    const int grid = (int)((a.lanes + B - 1) / B);
+    const bool chain = o.ref_exact != 0;
+    if ((st = pcf_timing_begin(ctx))) return st;
+    if (chain && (st = ensure_logtab(ctx, N))) return st;
+    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+    if (chain)
+        k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(),
+                                                  ctx->partials.as<pcf::CSum>(),
+                                                  ctx->counters.as<unsigned int>(),
+                                                  ctx->out.as<double>());
+    else
+        k_binom_lgamma<<<grid, B, 0, ctx->stream>>>(a, ctx->partials.as<pcf::CSum>(),
+                                                   ctx->counters.as<unsigned int>(),
+                                                   ctx->out.as<double>());
+    PCF_HIP_CHECK(ctx, hipGetLastError());
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if ((st = pcf_timing_end(ctx, 1, res))) return st;
+    // The reference returns exp(-r*T)*result (src/binom_embar.cpp:49).
+    res->mean = ctx->h_out[0];
+    res->price = std::exp(-r * T) * ctx->h_out[0];
+    res->n_paths = a.lanes;
+    res->n_steps = N;
+    // chain: the reference's 0.75 N^2 log-adds become table adds
+    // (sum over lanes of (N-i) + (N-i-1)); lgamma: ~20 flops per term.
+    double adds = 0.0;
+    if (chain) {
+        const double L = (double)a.until;
+        adds = 2.0 * (double)N * L - L * (L - 1.0) - L;  // sum_{i<until} (2N - 2i - 1)
+        if (N % 2 == 0) adds += (double)N - 1.0;         // middle lane
+    }
+    res->flops = chain ? adds + 24.0 * (double)a.lanes : 40.0 * (double)a.lanes;
+    res->bytes = chain ? 8.0 * adds : 0.0;
+    res->grid_blocks = grid;
+    res->block_size = B;
+    return PCF_OK;
+}

@@ -1,0 +1,233 @@
+// pcf_device.hpp -- device building blocks shared by the gfx950 pricing
+// kernels: counter-based Philox4x32-10, Box-Muller normals (FP32 on the
+// hardware transcendental unit, FP64 on the libm sequences), payoff, and
+// deterministic compensated FP64 reductions (wavefront shuffles + LDS, then a
+// last-arriving-block pass over the per-block partials).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcf {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC11).  Multipliers 0xD2511F53/0xCD9E8D57,
+// Weyl increments 0x9E3779B9/0xBB67AE85.
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint32_t lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        U4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Counter layout: (block of 4 normals, path lo, path hi, stream id).
+__device__ __forceinline__ U4 philox_block(uint64_t seed, uint32_t stream, uint64_t path,
+                                           uint32_t block) {
+    U4 c;
+    c.x = block;
+    c.y = (uint32_t)path;
+    c.z = (uint32_t)(path >> 32);
+    c.w = stream;
+    return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// ---------------------------------------------------------------------------
+// Box-Muller.  Uniform u = (x + 0.5) 2^-32 in (0,1); the pair (u0, u1) gives
+// z0 = R cos(2 pi u1), z1 = R sin(2 pi u1), R = sqrt(-2 ln u0).
+//
+// FP32: v_log_f32 is log2, v_sin_f32 / v_cos_f32 take their argument in
+// revolutions (sin(2 pi x)), so one instruction each; u is the FP32 rounding
+// of the FP64 uniform.
+__device__ __forceinline__ float u01_f32(uint32_t x) {
+    return fmaf((float)x, 0x1p-32f, 0x1p-33f);
+}
+
+__device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& z0, float& z1) {
+    const float u0 = u01_f32(a);
+    const float u1 = u01_f32(b);
+    // -2 ln u0 = -2 ln(2) log2(u0)
+    const float R = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u0));
+    z0 = R * __builtin_amdgcn_cosf(u1);
+    z1 = R * __builtin_amdgcn_sinf(u1);
+}
+
+__device__ __forceinline__ double u01_f64(uint32_t x) {
+    return ((double)x + 0.5) * 0x1p-32;
+}
+
+__device__ __forceinline__ void box_muller_f64(uint32_t a, uint32_t b, double& z0, double& z1) {
+    const double u0 = u01_f64(a);
+    const double u1 = u01_f64(b);
+    const double R = sqrt(-2.0 * log(u0));
+    double s, c;
+    sincos(6.283185307179586476925286766559 * u1, &s, &c);
+    z0 = R * c;
+    z1 = R * s;
+}
+
+template <typename Real>
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, Real& z0, Real& z1);
+template <>
+__device__ __forceinline__ void box_muller<float>(uint32_t a, uint32_t b, float& z0, float& z1) {
+    box_muller_f32(a, b, z0, z1);
+}
+template <>
+__device__ __forceinline__ void box_muller<double>(uint32_t a, uint32_t b, double& z0, double& z1) {
+    box_muller_f64(a, b, z0, z1);
+}
+
+// Four standard normals of (seed, stream, path, block).
+template <typename Real>
+__device__ __forceinline__ void normals4(uint64_t seed, uint32_t stream, uint64_t path,
+                                         uint32_t block, Real z[4]) {
+    const U4 x = philox_block(seed, stream, path, block);
+    box_muller<Real>(x.x, x.y, z[0], z[1]);
+    box_muller<Real>(x.z, x.w, z[2], z[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Fast FP32 exp on the hardware exp2 unit: exp(x) = 2^(x log2 e).  Kernels
+// fold log2(e) into their constants and call exp2 directly.
+__device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp2f(x); }
+
+template <typename Real>
+__device__ __forceinline__ Real payoff(Real S, Real E, Real pf) {
+    const Real v = pf * (S - E);
+    return v > Real(0) ? v : Real(0);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated FP64 sums.  value = s + c; TwoSum (Knuth) keeps the rounding
+// error of every add in c, so a sum of sums is accurate to ~1 ulp of the
+// total independent of the order of the partials.
+struct CSum {
+    double s, c;
+};
+
+__device__ __forceinline__ void csum_add(CSum& a, double v) {
+    const double t = a.s + v;
+    const double bp = t - a.s;
+    const double e = (a.s - (t - bp)) + (v - bp);
+    a.s = t;
+    a.c += e;
+}
+
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+    const double t = a.s + b.s;
+    const double bp = t - a.s;
+    const double e = (a.s - (t - bp)) + (b.s - bp);
+    CSum r;
+    r.s = t;
+    r.c = a.c + b.c + e;
+    return r;
+}
+
+// K compensated sums per thread.
+template <int K>
+struct Acc {
+    CSum v[K];
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k].s = v[k].c = 0.0;
+    }
+};
+
+template <int K>
+__device__ __forceinline__ void acc_merge_into(Acc<K>& a, const Acc<K>& b) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) a.v[k] = csum_merge(a.v[k], b.v[k]);
+}
+
+template <int K>
+__device__ __forceinline__ Acc<K> wave_reduce(Acc<K> a) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        Acc<K> o;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            o.v[k].s = __shfl_xor(a.v[k].s, off, kWave);
+            o.v[k].c = __shfl_xor(a.v[k].c, off, kWave);
+        }
+        acc_merge_into(a, o);
+    }
+    return a;
+}
+
+// Block reduction; the result is valid in thread 0.  `lds` holds
+// (blockDim/64) * K CSums.  Fixed order => deterministic for a launch shape.
+template <int K>
+__device__ __forceinline__ Acc<K> block_reduce(Acc<K> a, CSum* lds) {
+    a = wave_reduce<K>(a);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nwaves = (blockDim.x + kWave - 1) / kWave;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave * K + k] = a.v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nwaves; ++w) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) a.v[k] = csum_merge(a.v[k], lds[w * K + k]);
+        }
+    }
+    return a;
+}
+
+// Per-block partial write + "last block finishes" grid reduction.
+//   partials: gridDim.x * K CSums;  counter: one uint32, zero at launch and
+//   left zero on exit;  out: K doubles (value of each total sum).
+// Returns true in the (single) block that performed the final pass, whose
+// thread 0 then holds the totals in `tot`.
+template <int K>
+__device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partials,
+                                            unsigned int* __restrict__ counter, CSum* lds,
+                                            Acc<K>& tot) {
+    __shared__ bool is_last;
+    a = block_reduce<K>(a, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = a.v[k];
+        __threadfence();
+        const unsigned int prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    // Thread t folds partials t, t+B, t+2B, ... in order, then a block reduce.
+    Acc<K> r;
+    r.zero();
+    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) r.v[k] = csum_merge(r.v[k], partials[(size_t)b * K + k]);
+    }
+    __syncthreads();  // lds reuse
+    r = block_reduce<K>(r, lds);
+    if (threadIdx.x == 0) {
+        tot = r;
+        *counter = 0u;  // ready for the next launch
+    }
+    return true;
+}
+
+}  // namespace pcf

@@ -1,0 +1,280 @@
+"""Pins the oracle (oracle/oracle.cpp) before anything is checked against it.
+
+Each oracle function is checked against something that does not share its
+code: the reference's own kernels (17-digit fixtures written by
+tests/golden/make_golden.py from drivers built on the reference sources, and
+live when oracle/_ref exists), the reference's published CSV rows, Random123
+known answers and rocRAND's Philox, closed forms, and independent numpy
+restatements.
+"""
+import csv
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def _golden_rows():
+    with open(os.path.join(GOLDEN, "ref_kernels.csv")) as f:
+        return list(csv.DictReader(f))
+
+
+def _oracle_value(row):
+    L = O.lib()
+    pf = 1.0 if row["payoff"] == "call" else -1.0
+    S0, E, r, s, T = (float(row[k]) for k in ("S0", "E", "r", "sigma", "T"))
+    N, M, seed = int(row["N"]), int(row["M"]), int(row["seed"])
+    prog = row["prog"]
+    if prog == "mc_eur":
+        return L.orc_mc_eur_seed(S0, E, r, s, T, N, pf, seed)
+    if prog == "mc_asia":
+        return L.orc_mc_asia_seed(S0, E, r, s, T, N, M, pf, seed)
+    if prog == "mc_amer":
+        return L.orc_mc_amer_seed(S0, E, r, s, T, N, M, pf, seed, 1)
+    if prog == "binom_embar":
+        return L.orc_binom_embar(S0, E, r, s, T, N, pf, 1)
+    if prog == "binom_vanilla_eur":
+        return L.orc_binom_vanilla(S0, E, r, s, T, N, pf, 0)
+    if prog == "binom_vanilla_amer":
+        return L.orc_binom_vanilla(S0, E, r, s, T, N, pf, 1)
+    raise KeyError(prog)
+
+
+@pytest.mark.parametrize("row", _golden_rows(), ids=lambda r: f"{r['prog']}-{r['payoff']}-N{r['N']}-s{r['seed']}")
+def test_oracle_matches_reference_kernels_bitwise(row):
+    """The restatement reproduces the reference kernel's FP64 result exactly."""
+    if int(row["N"]) >= 100000 and row["prog"] == "binom_embar":
+        pytest.skip("N=1e5 chain takes ~1 min on the CPU; covered on the GPU against this fixture")
+    got = _oracle_value(row)
+    assert got == float(row["result"]), (got, row["result"])
+
+
+# results/results_binom_embar.csv (Serial rows): S0=100, E=110, r=0.02,
+# sigma=0.75, T=1, call; 10 significant digits.
+PUBLISHED_BINOM = [(100, 26.70698005), (800, 26.61900432), (1000, 26.60882645), (1600, 26.61872687),
+                   (3200, 26.61535275), (6400, 26.61230853), (8000, 26.61330534), (10000, 26.61316121)]
+
+
+@pytest.mark.parametrize("N,val", PUBLISHED_BINOM)
+def test_oracle_binom_matches_published_rows(N, val):
+    got = O.lib().orc_binom_embar(100, 110, 0.02, 0.75, 1, N, 1.0, 1)
+    assert float(f"{got:.10g}") == val
+
+
+def test_oracle_comb_special_cases_and_exact_small():
+    L = O.lib()
+    N = 30
+    assert L.orc_comb(N, 0) == 0.0 and L.orc_comb(N, N) == 0.0
+    assert L.orc_comb(N, 1) == math.log(N) and L.orc_comb(N, N - 1) == math.log(N)
+    for i in range(2, N - 1):
+        assert abs(L.orc_comb(N, i) - math.log(math.comb(N, i))) < 1e-12
+
+
+def test_philox_known_answers():
+    """Random123 kat_vectors for philox4x32-10."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kat:
+        assert O.philox_block(ctr, key) == want
+
+
+def test_philox_matches_rocrand_header():
+    exe = O.ref_bin("rocrand_kat")
+    if exe is None:
+        pytest.skip("oracle/_ref/rocrand_kat not built (needs hipcc)")
+    rng = np.random.default_rng(5)
+    for _ in range(16):
+        v = [int(x) for x in rng.integers(0, 2**32, size=6, dtype=np.uint64)]
+        out = subprocess.run([exe] + ["%x" % x for x in v], check=True, capture_output=True, text=True).stdout
+        want = tuple(int(w, 16) for w in out.split())
+        assert O.philox_block(v[2:6], v[0:2]) == want
+
+
+def test_box_muller_stream_moments():
+    z = O.philox_normals(123, 2, 0, 1 << 15, 8).ravel()
+    assert abs(z.mean()) < 4.0 / math.sqrt(z.size)
+    assert abs(z.var() - 1.0) < 6.0 * math.sqrt(2.0 / z.size)
+    # Kolmogorov-Smirnov against N(0,1)
+    from scipy import stats
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    # uniforms: block 0 of path 0 from the raw Philox words
+    x = O.philox_block((0, 0, 0, 2), (123, 0))
+    u0, u1 = ((x[0] + 0.5) * 2.0**-32, (x[1] + 0.5) * 2.0**-32)
+    R = math.sqrt(-2.0 * math.log(u0))
+    assert abs(z[0] - R * math.cos(2 * math.pi * u1)) < 1e-15
+    assert abs(z[1] - R * math.sin(2 * math.pi * u1)) < 1e-15
+
+
+def test_black_scholes_known_values():
+    L = O.lib()
+    c = L.orc_bs_price(100, 100, 0.05, 0.2, 1, 1.0)
+    p = L.orc_bs_price(100, 100, 0.06, 0.2, 1, -1.0)
+    assert abs(c - 10.450583572185565) < 1e-12
+    assert abs(p - 5.166002511) < 1e-9
+    # put-call parity
+    c2 = L.orc_bs_price(100, 100, 0.06, 0.2, 1, 1.0)
+    assert abs((c2 - p) - (100 - 100 * math.exp(-0.06))) < 1e-12
+
+
+def test_geometric_asian_closed_form():
+    L = O.lib()
+    S0, K, r, s, T = 100, 100, 0.05, 0.25, 1.0
+    # M = 1 is Black-Scholes
+    assert abs(L.orc_geo_asian(S0, K, r, s, T, 1, 1.0) - L.orc_bs_price(S0, K, r, s, T, 1.0)) < 1e-12
+    # mean / variance of the average log price from the double sum of min(i,j)
+    M = 252
+    dt = T / M
+    i = np.arange(1, M + 1)
+    var = s * s * dt * np.minimum.outer(i, i).sum() / M**2
+    assert abs(var - s * s * T * (M + 1) * (2 * M + 1) / (6 * M * M)) < 1e-14
+    assert abs(L.orc_geo_asian(S0, K, r, s, T, M, 1.0) - 6.551716578) < 1e-8
+    # continuous limit: sigma_a = s/sqrt(3), b = (r - s^2/6)/2
+    sa = s / math.sqrt(3)
+    b = 0.5 * (r - s * s / 6)
+    d1 = (math.log(S0 / K) + (b + 0.5 * sa * sa) * T) / (sa * math.sqrt(T))
+    d2 = d1 - sa * math.sqrt(T)
+    from scipy.stats import norm
+    cont = S0 * math.exp((b - r) * T) * norm.cdf(d1) - K * math.exp(-r * T) * norm.cdf(d2)
+    assert abs(cont - 6.529835446) < 1e-8
+    assert abs(L.orc_geo_asian(S0, K, r, s, T, 200000, 1.0) - cont) < 1e-4
+
+
+def test_inverse3_against_lu():
+    rng = np.random.default_rng(3)
+    L = O.lib()
+    for _ in range(50):
+        A = rng.normal(size=(3, 3))
+        A = A @ A.T + 0.1 * np.eye(3)
+        y = rng.normal(size=3)
+        c = np.empty(3)
+        assert L.orc_inverse3_solve(O.ptr(np.ascontiguousarray(A)), O.ptr(y), O.ptr(c)) == 1
+        assert np.allclose(c, np.linalg.solve(A, y), rtol=1e-10, atol=1e-12)
+    S = np.ones((3, 3))
+    c = np.empty(3)
+    assert L.orc_inverse3_solve(O.ptr(S), O.ptr(np.ones(3)), O.ptr(c)) == 0  # det <= 0
+
+
+def test_cholesky_residual_and_equicorrelation():
+    L = O.lib()
+    rng = np.random.default_rng(9)
+    for d in (2, 8, 32):
+        A = rng.normal(size=(d, d))
+        C = A @ A.T + 0.5 * np.eye(d)
+        Dm = np.diag(1 / np.sqrt(np.diag(C)))
+        C = Dm @ C @ Dm
+        Lf = np.empty((d, d))
+        assert L.orc_cholesky(d, O.ptr(np.ascontiguousarray(C)), O.ptr(Lf)) == 1
+        assert np.abs(Lf @ Lf.T - C).max() < 1e-13
+        assert np.allclose(np.triu(Lf, 1), 0)
+    d, rho = 8, 0.3
+    C = np.full((d, d), rho) + (1 - rho) * np.eye(d)
+    Lf = np.empty((d, d))
+    L.orc_cholesky(d, O.ptr(C), O.ptr(Lf))
+    assert abs(Lf[0, 0] - 1) < 1e-15 and np.allclose(Lf[1:, 0], rho)
+    assert L.orc_cholesky(2, O.ptr(np.ones((2, 2))), O.ptr(np.empty((2, 2)))) == 0
+
+
+def test_basket_reduces_to_mc_eur():
+    """d = 1 is the single-asset estimator on the same normals, bit for bit;
+    a factor whose first column is all ones (rho -> 1) prices the same."""
+    L = O.lib()
+    N = 20000
+    z = O.normals_mt(77, 1.0, N)
+    one = np.ones(1)
+    b1 = L.orc_basket_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, 1, O.ptr(one), O.ptr(z), None, None, None)
+    e1 = L.orc_mc_eur_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, O.ptr(z))
+    assert b1 == e1
+    d = 4
+    Lf = np.zeros((d, d))
+    Lf[:, 0] = 1.0
+    Z = np.zeros((N, d))
+    Z[:, 0] = z
+    Z[:, 1:] = O.normals_mt(78, 1.0, N * (d - 1)).reshape(N, d - 1)
+    bd = L.orc_basket_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, d, O.ptr(Lf), O.ptr(np.ascontiguousarray(Z)),
+                        None, None, None)
+    assert abs(bd - e1) < 1e-13 * e1
+
+
+def _lsm_numpy(S0, E, r, sigma, T, N, M, pf, w, quirk):
+    """Independent vectorised restatement of src/mc_amer.cpp:5-114 (numpy)."""
+    dt = T / M
+    incr = (r - 0.5 * sigma * sigma) * dt
+    P = np.empty((M + 1, N))
+    P[0] = S0
+    W = w.reshape(N // 2, M)
+    for m in range(1, M + 1):
+        P[m, :N // 2] = P[m - 1, :N // 2] * np.exp(incr + sigma * W[:, m - 1])
+        P[m, N // 2:] = P[m - 1, N // 2:] * np.exp(incr - sigma * W[:, m - 1])
+    pay = lambda S: np.maximum(pf * (S - E), 0.0)
+    when = np.full(N, M)
+    st = pay(P[M])
+    idx = np.arange(N)
+    for m in range(M - 1, 0, -1):
+        itm = pay(P[m]) > 0
+        if not itm.any():
+            continue
+        x = P[m, itm] - E
+        y = np.exp(-r * dt * (when[itm] - m)) * pay(P[when[itm], idx[itm]])
+        if itm.sum() <= 2:
+            pv = pay(P[m, itm])
+            ex = pv > y
+            when[idx[itm][ex]] = m
+            st[idx[itm][ex]] = pv[ex]
+            continue
+        X = np.stack([np.ones_like(x), x, x * x], 1)
+        coef = np.linalg.solve(X.T @ X, X.T @ y)
+        yhat = coef[0] + coef[1] * x + coef[2] * x * x
+        pv = pay(x) if quirk else pay(P[m, itm])
+        ex = pv > yhat
+        when[idx[itm][ex]] = m
+        st[idx[itm][ex]] = pv[ex]
+    return max(pay(np.array(S0)).item(), (np.exp(-r * when * dt) * st).sum() / N)
+
+
+@pytest.mark.parametrize("pf,quirk", [(1.0, 1), (-1.0, 1), (-1.0, 0)])
+def test_lsm_against_numpy_restatement(pf, quirk):
+    N, M = 512, 10
+    w = O.normals_mt(31, math.sqrt(1.0 / M), (N // 2) * M)
+    got = O.lib().orc_mc_amer_z(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), quirk, None)
+    want = _lsm_numpy(100, 100, 0.06, 0.2, 1.0, N, M, pf, w, quirk)
+    assert abs(got - want) < 1e-10 * abs(want)
+
+
+def test_lsm_production_put_brackets():
+    """Corrected LSM put: above the European put, near the reference lattice."""
+    L = O.lib()
+    N, M = 1 << 14, 50
+    w = np.empty((N // 2) * M)
+    L.orc_philox_amer_normals(11, N, M, 1.0, O.ptr(w))
+    v = L.orc_mc_amer_z(100, 100, 0.06, 0.2, 1.0, N, M, -1.0, O.ptr(w), 0, None)
+    assert v > L.orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)
+    assert abs(v - 5.798877611783797) < 0.1  # binom_vanilla_amer N=2000 fixture
+
+
+def test_lattice_european_matches_closed_sum():
+    L = O.lib()
+    a = L.orc_binom_vanilla(100, 100, 0.05, 0.2, 1.0, 1000, 1.0, 0)
+    b = L.orc_binom_embar(100, 100, 0.05, 0.2, 1.0, 1000, 1.0, 1)
+    assert abs(a - b) < 1e-9
+    assert abs(b - 10.4490686590) < 1e-9
+
+
+def test_philox_production_estimators_statistics():
+    L = O.lib()
+    import ctypes as C
+    se = C.c_double()
+    v = L.orc_philox_mc_eur(100, 100, 0.05, 0.2, 1.0, 1 << 18, 1.0, 3, C.byref(se))
+    assert abs(v - 10.450583572185565) < 4 * se.value
+    geo = C.c_double()
+    a = L.orc_philox_mc_asia(100, 100, 0.05, 0.25, 1.0, 4096, 252, 1.0, 3, C.byref(se), C.byref(geo))
+    assert a > geo.value  # arithmetic average dominates the geometric one path by path
+    assert abs(geo.value - 6.551716578) < 0.5
