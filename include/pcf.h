@@ -81,7 +81,9 @@ typedef struct pcf_opts {
     int32_t block_size;      /* threads per block; 0 = tuned default */
     int32_t items_per_thread;/* paths (or path groups) per thread; 0 = tuned default */
     int32_t* when_out;       /* mc_amer: optional host array of N exercise dates */
-    int32_t reserved[6];
+    int32_t kernel_variant;  /* 0 = automatic; mc_eur_multi FP32: 1 = VALU FMA kernel,
+                                2 = MFMA kernel (needs 16 <= assets <= 32) */
+    int32_t reserved[5];
 } pcf_opts;
 
 typedef struct pcf_result {
@@ -100,7 +102,9 @@ typedef struct pcf_result {
     double flops;        /* algorithmic flop count of the dominant kernel */
     double bytes;        /* algorithmic HBM bytes of the dominant kernel */
     int32_t lsm_small_itm; /* mc_amer: dates with 1..2 in-the-money paths */
-    int32_t lsm_fallback;  /* mc_amer: dates whose Gram solve fell back to a smaller basis */
+    int32_t lsm_fallback;  /* mc_amer: dates whose Gram solve fell back to a smaller basis;
+                              mc_eur_multi: 1 if the correlation factor came from the
+                              eigen-decomposition (not positive definite) */
     int32_t grid_blocks;   /* launch shape of the dominant kernel */
     int32_t block_size;
     int32_t reserved[4];

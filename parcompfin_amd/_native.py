@@ -58,7 +58,8 @@ class PcfOpts(C.Structure):
         ("block_size", C.c_int32),
         ("items_per_thread", C.c_int32),
         ("when_out", C.POINTER(C.c_int32)),
-        ("reserved", C.c_int32 * 6),
+        ("kernel_variant", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

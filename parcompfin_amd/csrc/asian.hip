@@ -196,7 +196,7 @@ extern "C" int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double s
     a.mu = (r - 0.5 * sigma * sigma) * dt;
     a.vol = sigma * sdt;
     a.h_c1 = 1 + r * dt / 2;
-    a.h_mu = (r - std::pow(sigma, 2) / 2) * dt;
+    a.h_mu = (r - libm_pow(sigma, 2) / 2) * dt;
     a.sigma = sigma;
     a.S0f = (float)S0;
     a.Ef = (float)E;

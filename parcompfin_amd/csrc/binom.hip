@@ -157,9 +157,9 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
 
     // Tree parameters exactly as src/binom_embar.cpp:16-24 (host, glibc).
     const double dt = (double)T / (double)N;
-    const double beta = 0.5 * (std::exp(-r * dt) + std::exp((r + std::pow(sigma, 2)) * dt));
-    const double u = beta + std::sqrt(std::pow(beta, 2) - 1);
-    const double d = beta - std::sqrt(std::pow(beta, 2) - 1);
+    const double beta = 0.5 * (std::exp(-r * dt) + std::exp((r + libm_pow(sigma, 2)) * dt));
+    const double u = beta + std::sqrt(libm_pow(beta, 2) - 1);
+    const double d = beta - std::sqrt(libm_pow(beta, 2) - 1);
     const double R = std::exp(r * dt);
     const double p = (R - d) / (u - d);
     const double q = 1 - p;

@@ -78,6 +78,7 @@ int pcf_ctx_destroy(pcf_ctx* ctx) {
     if (!ctx) return PCF_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    pcf_lsm_release(ctx);
     for (DevBuf* b : {&ctx->partials, &ctx->counters, &ctx->out, &ctx->normals, &ctx->logtab,
                       &ctx->prefix, &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef,
                       &ctx->basket_L, &ctx->scratch})

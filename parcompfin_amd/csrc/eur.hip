@@ -109,7 +109,7 @@ extern "C" int pcf_mc_eur(pcf_ctx* ctx, double S0, double E, double r, double si
     a.sigma = sigma;
     a.mu = (r - 0.5 * sigma * sigma) * T;
     a.vol = sigma * std::sqrt(T);
-    a.href_mu = (r - std::pow(sigma, 2) / 2) * T;
+    a.href_mu = (r - libm_pow(sigma, 2) / 2) * T;
     const double log2e = 1.4426950408889634;
     a.S0f = (float)S0;
     a.Ef = (float)E;

@@ -1,0 +1,678 @@
+// lsm.hip -- American option by Longstaff-Schwartz on an antithetic path store.
+// Reference: src/mc_amer.cpp:5-114, paths from pathsfinder
+// (include/common.h:168-208), 3x3 solve include/common.h:98-141.
+//
+// Device data (HBM, structure of arrays, coalesced along the path index n):
+//   store[(m-1)*N + n], m = 1..M  path values (time-major like the reference's
+//                                 paths[m][n]; row 0 = S0 is not stored)
+//   when[n] (int32), cf[n], st[n]  per-path exercise date, true payoff at that
+//                                 date (the reference's y gathers it from
+//                                 paths[when][n]; here it is carried), and the
+//                                 booked cash flow (differs from cf only under
+//                                 the reference's payoff(S-E,E) exercise test)
+//   coef[m] = {flag, c0, c1, c2}  regression result per date
+// Launch sequence (one stream, captured once per shape into a hipGraph):
+//   k_lsm_paths        store + state init + European control + sums of date M-1
+//   k_lsm_step(m)      exercise decision at m, then sums of date m-1   (m = M-1..2)
+//   k_lsm_final        decision at date 1, then sum of discounted cash flows
+// Each sums pass reduces 8 compensated FP64 sums (n, x, x^2, x^3, x^4, y,
+// xy, x^2 y) through the last-arriving block, which solves the normal
+// equations in FP64 and writes coef for the next launch.
+//
+// Arithmetic forms:
+//  * REF (ref_exact): FP64, basis {1, S-E, (S-E)^2}, cofactor inverse with the
+//    reference's operation order, exercise test payoff(S-E, E)
+//    (src/mc_amer.cpp:100; wrong for puts, reproduced for parity), det <= 0
+//    flagged (the reference throws).
+//  * production: basis {1, x, x^2} with x = S/E (same span as {1, S, S^2}),
+//    FP64 sums and a 3x3 Cholesky solve with fallback to {1, x} then {1},
+//    exercise test payoff(S, E); FP32 or FP64 path store.
+// Dates with 1-2 in-the-money paths compare payoff with the discounted cash
+// flow directly (src/mc_amer.cpp:75-83), in both forms.
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "pcf_internal.hpp"
+
+namespace {
+
+enum { kPhilox = 0, kHost = 1 };
+enum { kFlagSkip = 0, kFlagSmall = 1, kFlagRegress = 2, kFlagSingular = 3, kFlagReduced = 4 };
+
+struct LsmParams {
+    double S0, E, pf, invE;
+    double nrdt;        // (-r)*dt            reference discount of y
+    double nr, dt;      // -r, dt             reference final discount exp((-r*when)*dt)
+    double h_mu, sigma; // (r-0.5*sigma*sigma)*dt, sigma   reference store recurrence
+    double mu, vol;     // (r - s^2/2) dt, s sqrt(dt)      production FP64
+    float mu2f, vol2f, S0f, Ef, pff, pad;  // production FP32 (x log2 e)
+    int64_t N, half;
+    int32_t M, pad2;
+    uint64_t seed;
+};
+
+struct LsmBufs {
+    void* store;
+    int32_t* when;
+    void* cf;
+    void* st;
+    double* coef;  // 4 per date, dates 0..M
+    double* disc;  // exp(-r dt k), k = 0..M
+    const double* hw;  // host normals (verification)
+    pcf::CSum* partials;
+    unsigned int* counter;
+    double* out;
+    const LsmParams* P;
+};
+
+template <typename Real>
+__device__ __forceinline__ Real pay(Real S, Real E, Real pf) {
+    return pcf::payoff<Real>(S, E, pf);
+}
+
+// Regression sums of one path at date m into acc[0..7].
+template <typename Real, bool REF>
+__device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double* __restrict__ disc,
+                                               int m, Real S, int32_t when, Real cf, double acc[8]) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    if (pv > 0.0) {
+        if (REF) {
+#pragma clang fp contract(off)
+            const double x = (double)S - P.E;
+            const double y = exp(P.nrdt * ((double)when - (double)m)) * (double)cf;
+            acc[0] += 1.0;
+            acc[1] += x;
+            acc[2] += x * x;
+            acc[3] += x * x * x;
+            acc[4] += x * x * x * x;
+            acc[5] += y;
+            acc[6] += y * x;
+            acc[7] += y * x * x;
+        } else {
+            const double x = (double)S * P.invE;
+            const double y = disc[when - m] * (double)cf;
+            const double x2 = x * x;
+            acc[0] += 1.0;
+            acc[1] += x;
+            acc[2] += x2;
+            acc[3] += x2 * x;
+            acc[4] += x2 * x2;
+            acc[5] += y;
+            acc[6] += y * x;
+            acc[7] += y * x2;
+        }
+    }
+}
+
+// Normal equations -> coef[date].  Runs in one thread.
+template <bool REF>
+__device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int date) {
+    double g[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = tot.v[k].s + tot.v[k].c;
+    double* c = coef + 4 * date;
+    const double n = g[0];
+    c[1] = c[2] = c[3] = 0.0;
+    if (n == 0.0) {
+        c[0] = kFlagSkip;
+        return;
+    }
+    if (n <= 2.0) {
+        c[0] = kFlagSmall;
+        return;
+    }
+    if (REF) {
+#pragma clang fp contract(off)
+        // include/common.h:98-124 then :128-141, same order.
+        double x[3][3] = {{n, g[1], g[2]}, {g[1], g[2], g[3]}, {g[2], g[3], g[4]}};
+        const double y[3] = {g[5], g[6], g[7]};
+        double det = 0;
+        for (int i = 0; i < 3; ++i)
+            det += (x[0][i] * (x[1][(i + 1) % 3] * x[2][(i + 2) % 3] - x[1][(i + 2) % 3] * x[2][(i + 1) % 3]));
+        if (det <= 0) {
+            c[0] = kFlagSingular;
+            return;
+        }
+        double inv[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                inv[j][i] = ((x[(j + 1) % 3][(i + 1) % 3] * x[(j + 2) % 3][(i + 2) % 3]) -
+                             (x[(j + 1) % 3][(i + 2) % 3] * x[(j + 2) % 3][(i + 1) % 3])) / det;
+        for (int i = 0; i < 3; ++i) {
+            double mm = 0;
+            for (int j = 0; j < 3; ++j) mm += inv[i][j] * y[j];
+            c[1 + i] = mm;
+        }
+        c[0] = kFlagRegress;
+        return;
+    }
+    // Production: FP64 Cholesky of [[n,s1,s2],[s1,s2,s3],[s2,s3,s4]].
+    const double eps = 1e-13;
+    const double g00 = n, g10 = g[1], g20 = g[2], g11 = g[2], g21 = g[3], g22 = g[4];
+    const double b0 = g[5], b1 = g[6], b2 = g[7];
+    const double l00 = sqrt(g00);
+    const double l10 = g10 / l00, l20 = g20 / l00;
+    const double p11 = g11 - l10 * l10;
+    if (p11 > eps * g11) {
+        const double l11 = sqrt(p11);
+        const double l21 = (g21 - l20 * l10) / l11;
+        const double p22 = g22 - l20 * l20 - l21 * l21;
+        if (p22 > eps * g22) {
+            const double l22 = sqrt(p22);
+            const double y0 = b0 / l00;
+            const double y1 = (b1 - l10 * y0) / l11;
+            const double y2 = (b2 - l20 * y0 - l21 * y1) / l22;
+            const double x2 = y2 / l22;
+            const double x1 = (y1 - l21 * x2) / l11;
+            const double x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+            c[1] = x0;
+            c[2] = x1;
+            c[3] = x2;
+            c[0] = kFlagRegress;
+            return;
+        }
+        // {1, x}
+        const double y0 = b0 / l00;
+        const double y1 = (b1 - l10 * y0) / l11;
+        const double x1 = y1 / l11;
+        c[1] = (y0 - l10 * x1) / l00;
+        c[2] = x1;
+        c[0] = kFlagReduced;
+        return;
+    }
+    c[1] = b0 / n;  // {1}: the mean continuation value
+    c[0] = kFlagReduced;
+}
+
+// Exercise decision at date m for one path.  Returns true if state changed.
+template <typename Real, bool REF>
+__device__ __forceinline__ bool lsm_decide(const LsmParams& P, const double* __restrict__ disc,
+                                           const double* __restrict__ cm, int m, Real S,
+                                           int32_t& when, Real& cf, Real& st) {
+    const int flag = (int)cm[0];
+    if (flag != kFlagSmall && flag != kFlagRegress && flag != kFlagReduced) return false;
+    const Real pvr = pay<Real>(S, (Real)P.E, (Real)P.pf);
+    const double pv = (double)pvr;
+    if (!(pv > 0.0)) return false;
+    if (REF) {
+#pragma clang fp contract(off)
+        const double x = (double)S - P.E;
+        if (flag == kFlagSmall) {
+            const double y = exp(P.nrdt * ((double)when - (double)m)) * (double)cf;
+            if (pv > y) {
+                when = m;
+                cf = pvr;
+                st = pvr;
+                return true;
+            }
+            return false;
+        }
+        // The regression pass skips x == -1, the reference's "not in the money"
+        // marker (src/mc_amer.cpp:98).
+        if (x == -1.0) return false;
+        const double yhat = cm[1] + cm[2] * x + cm[3] * (x * x);
+        const double test = pay<double>(x, P.E, P.pf);  // payoff(x[i],E,payoff_fun)
+        if (test > yhat) {
+            when = m;
+            cf = pvr;
+            st = (Real)test;
+            return true;
+        }
+        return false;
+    }
+    if (flag == kFlagSmall) {
+        const double y = disc[when - m] * (double)cf;
+        if (pv > y) {
+            when = m;
+            cf = pvr;
+            st = pvr;
+            return true;
+        }
+        return false;
+    }
+    const double x = (double)S * P.invE;
+    const double yhat = cm[1] + x * (cm[2] + x * cm[3]);
+    if (pv > yhat) {
+        when = m;
+        cf = pvr;
+        st = pvr;
+        return true;
+    }
+    return false;
+}
+
+// --- k_lsm_paths ------------------------------------------------------------
+// Verification store: host normals w[n*M + (m-1)] (pathsfinder's draw order),
+// the reference's recurrence in FP64 without contraction.
+template <bool REF>
+__global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
+    using Real = double;
+    __shared__ pcf::CSum lds[4 * 10];
+    const LsmParams& P = *b.P;
+    Real* __restrict__ store = (Real*)b.store;
+    Real* __restrict__ cfv = (Real*)b.cf;
+    Real* __restrict__ stv = (Real*)b.st;
+    const int64_t N = P.N, half = P.half;
+    const int M = P.M;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double e0 = 0.0, e1 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < half; n += stride) {
+        Real Sp = (Real)P.S0, Sm = (Real)P.S0, Sp_prev = Sp, Sm_prev = Sm;
+        for (int m = 1; m <= M; ++m) {
+            Sp_prev = Sp;
+            Sm_prev = Sm;
+            {
+#pragma clang fp contract(off)
+                const double w = b.hw[n * M + (m - 1)];
+                // paths[m][n] = paths[m-1][n]*exp((r-0.5*sigma*sigma)*dt +/- sigma*w)
+                Sp = Sp * exp(P.h_mu + P.sigma * w);
+                Sm = Sm * exp(P.h_mu - P.sigma * w);
+            }
+            store[(int64_t)(m - 1) * N + n] = Sp;
+            store[(int64_t)(m - 1) * N + n + half] = Sm;
+        }
+        const Real cp = pay<Real>(Sp, (Real)P.E, (Real)P.pf);
+        const Real cm_ = pay<Real>(Sm, (Real)P.E, (Real)P.pf);
+        b.when[n] = M;
+        b.when[n + half] = M;
+        cfv[n] = cp;
+        cfv[n + half] = cm_;
+        if (REF) {
+            stv[n] = cp;
+            stv[n + half] = cm_;
+        }
+        e0 += (double)cp + (double)cm_;
+        e1 += (double)cp * (double)cp + (double)cm_ * (double)cm_;
+        if (M >= 2) {
+            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
+            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+        }
+    }
+    pcf::Acc<10> a, tot;
+    a.v[0] = {e0, 0.0};
+    a.v[1] = {e1, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+        b.out[0] = tot.v[0].s + tot.v[0].c;
+        b.out[1] = tot.v[1].s + tot.v[1].c;
+        if (M >= 2) {
+            pcf::Acc<8> g;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g.v[k] = tot.v[2 + k];
+            lsm_solve<REF>(g, b.coef, M - 1);
+        }
+    }
+}
+
+// Production store: one Philox block (pair n, block (m-1)/4) feeds 4 dates.
+template <typename Real, bool REF>
+__global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
+    __shared__ pcf::CSum lds[4 * 10];
+    const LsmParams& P = *b.P;
+    Real* __restrict__ store = (Real*)b.store;
+    Real* __restrict__ cfv = (Real*)b.cf;
+    Real* __restrict__ stv = (Real*)b.st;
+    const int64_t N = P.N, half = P.half;
+    const int M = P.M;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double e0 = 0.0, e1 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < half; n += stride) {
+        Real Sp = (Real)P.S0, Sm = (Real)P.S0, Sp_prev = Sp, Sm_prev = Sm;
+        for (int m0 = 1; m0 <= M; m0 += 4) {
+            Real z[4];
+            pcf::normals4<Real>(P.seed, PCF_STREAM_AMER, (uint64_t)n, (uint32_t)((m0 - 1) >> 2), z);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + q;
+                if (m <= M) {
+                    Sp_prev = Sp;
+                    Sm_prev = Sm;
+                    if constexpr (sizeof(Real) == 4) {
+                        Sp *= pcf::exp2_f32(fmaf(P.vol2f, z[q], P.mu2f));
+                        Sm *= pcf::exp2_f32(fmaf(-P.vol2f, z[q], P.mu2f));
+                    } else {
+                        Sp *= exp(fma(P.vol, (double)z[q], P.mu));
+                        Sm *= exp(fma(-P.vol, (double)z[q], P.mu));
+                    }
+                    store[(int64_t)(m - 1) * N + n] = Sp;
+                    store[(int64_t)(m - 1) * N + n + half] = Sm;
+                }
+            }
+        }
+        const Real cp = pay<Real>(Sp, (Real)P.E, (Real)P.pf);
+        const Real cm_ = pay<Real>(Sm, (Real)P.E, (Real)P.pf);
+        b.when[n] = M;
+        b.when[n + half] = M;
+        cfv[n] = cp;
+        cfv[n + half] = cm_;
+        if (REF) {
+            stv[n] = cp;
+            stv[n + half] = cm_;
+        }
+        e0 += (double)cp + (double)cm_;
+        e1 += (double)cp * (double)cp + (double)cm_ * (double)cm_;
+        if (M >= 2) {
+            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
+            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+        }
+    }
+    pcf::Acc<10> a, tot;
+    a.v[0] = {e0, 0.0};
+    a.v[1] = {e1, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+        b.out[0] = tot.v[0].s + tot.v[0].c;
+        b.out[1] = tot.v[1].s + tot.v[1].c;
+        if (M >= 2) {
+            pcf::Acc<8> g;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g.v[k] = tot.v[2 + k];
+            lsm_solve<REF>(g, b.coef, M - 1);
+        }
+    }
+}
+
+// --- k_lsm_step(m): decide at m, sums at m-1 --------------------------------
+template <typename Real, bool REF>
+__global__ __launch_bounds__(256) void k_lsm_step(LsmBufs b, int m) {
+    __shared__ pcf::CSum lds[4 * 8];
+    const LsmParams& P = *b.P;
+    const Real* __restrict__ store = (const Real*)b.store;
+    Real* __restrict__ cfv = (Real*)b.cf;
+    Real* __restrict__ stv = (Real*)b.st;
+    const int64_t N = P.N;
+    const double* __restrict__ cm = b.coef + 4 * m;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+        int32_t when = b.when[n];
+        Real cf = cfv[n];
+        Real st = REF ? stv[n] : cf;
+        const Real S = store[(int64_t)(m - 1) * N + n];
+        if (lsm_decide<Real, REF>(P, b.disc, cm, m, S, when, cf, st)) {
+            b.when[n] = when;
+            cfv[n] = cf;
+            if (REF) stv[n] = st;
+        }
+        const Real Sprev = store[(int64_t)(m - 2) * N + n];
+        lsm_accumulate<Real, REF>(P, b.disc, m - 1, Sprev, when, cf, acc);
+    }
+    pcf::Acc<8> a, tot;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<8>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0)
+        lsm_solve<REF>(tot, b.coef, m - 1);
+}
+
+// --- k_lsm_final: decide at date 1 (if M >= 2), sum discounted cash flows ----
+template <typename Real, bool REF>
+__global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
+    __shared__ pcf::CSum lds[4 * 2];
+    const LsmParams& P = *b.P;
+    const Real* __restrict__ store = (const Real*)b.store;
+    Real* __restrict__ cfv = (Real*)b.cf;
+    Real* __restrict__ stv = (Real*)b.st;
+    const int64_t N = P.N;
+    const int M = P.M;
+    const double* __restrict__ cm = b.coef + 4;
+    double s = 0.0, s2 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+        int32_t when = b.when[n];
+        Real cf = cfv[n];
+        Real st = REF ? stv[n] : cf;
+        if (M >= 2) {
+            const Real S = store[n];  // row of date 1
+            if (lsm_decide<Real, REF>(P, b.disc, cm, 1, S, when, cf, st)) {
+                b.when[n] = when;
+                cfv[n] = cf;
+                if (REF) stv[n] = st;
+            }
+        }
+        double v;
+        if (REF) {
+#pragma clang fp contract(off)
+            // if(exercise_st[n]!=0) result+=exp(-r*exercise_when[n]*dt)*exercise_st[n];
+            v = ((double)st != 0.0) ? exp(P.nr * (double)when * P.dt) * (double)st : 0.0;
+        } else {
+            v = b.disc[when] * (double)st;
+        }
+        s += v;
+        s2 += v * v;
+    }
+    pcf::Acc<2> a, tot;
+    a.v[0] = {s, 0.0};
+    a.v[1] = {s2, 0.0};
+    if (pcf::grid_reduce<2>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+        b.out[2] = tot.v[0].s + tot.v[0].c;
+        b.out[3] = tot.v[1].s + tot.v[1].c;
+    }
+}
+
+// ---- host: buffers, graph cache --------------------------------------------
+struct GraphKey {
+    int64_t N;
+    int M, real, src, ref, grid_paths, grid_step, B;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(N, M, real, src, ref, grid_paths, grid_step, B) <
+               std::tie(o.N, o.M, o.real, o.src, o.ref, o.grid_paths, o.grid_step, o.B);
+    }
+};
+
+struct LsmGraph {
+    hipGraphExec_t exec = nullptr;
+    void* store_ptr = nullptr;  // buffers the graph was captured with
+    void* state_ptr = nullptr;
+    void* coef_ptr = nullptr;
+    void* part_ptr = nullptr;
+    void* norm_ptr = nullptr;
+};
+
+std::map<pcf_ctx*, std::map<GraphKey, LsmGraph>>& graph_cache() {
+    static std::map<pcf_ctx*, std::map<GraphKey, LsmGraph>> c;
+    return c;
+}
+std::mutex& graph_mu() {
+    static std::mutex m;
+    return m;
+}
+
+template <typename Real, int SRC, bool REF>
+void enqueue_lsm(const LsmBufs& b, int M, int grid_paths, int grid_step, int B, hipStream_t s) {
+    if constexpr (SRC == kHost)
+        k_lsm_paths_host<REF><<<grid_paths, B, 0, s>>>(b);
+    else
+        k_lsm_paths_philox<Real, REF><<<grid_paths, B, 0, s>>>(b);
+    for (int m = M - 1; m >= 2; --m) k_lsm_step<Real, REF><<<grid_step, B, 0, s>>>(b, m);
+    k_lsm_final<Real, REF><<<grid_step, B, 0, s>>>(b);
+}
+
+}  // namespace
+
+// Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
+void pcf_lsm_release(pcf_ctx* ctx) {
+    std::lock_guard<std::mutex> lk(graph_mu());
+    auto it = graph_cache().find(ctx);
+    if (it == graph_cache().end()) return;
+    for (auto& kv : it->second)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    graph_cache().erase(it);
+}
+
+extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                           int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts_in,
+                           pcf_result* res) {
+    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const pcf_opts o = pcf_opts_or_default(opts_in);
+    int st = pcf_check_common(ctx, S0, E, r, sigma, T, N, payoff_fun);
+    if (st) return st;
+    if ((st = pcf_check_opts(ctx, &o))) return st;
+    if (N % 2 != 0)  // include/common.h:180
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "N needs to be divisible by 2 for finding paths");
+    if (M < 1 || M > 100000) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "M must be in [1, 100000]");
+    if (!(E > 0)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "E must be positive");
+    PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    *res = pcf_result{};
+
+    const int src = o.rng == PCF_RNG_HOST ? kHost : kPhilox;
+    const bool ref = o.ref_exact != 0;
+    const bool f64 = ref || src == kHost || o.precision == PCF_FP64;
+    const size_t rs = f64 ? sizeof(double) : sizeof(float);
+
+    // Parameter block (host copy -> device), discount table, buffers.
+    const double dt = T / M;
+    LsmParams P{};
+    P.S0 = S0;
+    P.E = E;
+    P.pf = payoff_fun;
+    P.invE = 1.0 / E;
+    P.nrdt = -r * dt;
+    P.nr = -r;
+    P.dt = dt;
+    P.h_mu = (r - 0.5 * sigma * sigma) * dt;
+    P.sigma = sigma;
+    P.mu = (r - 0.5 * sigma * sigma) * dt;
+    P.vol = sigma * std::sqrt(dt);
+    const double log2e = 1.4426950408889634;
+    P.mu2f = (float)(P.mu * log2e);
+    P.vol2f = (float)(P.vol * log2e);
+    P.S0f = (float)S0;
+    P.Ef = (float)E;
+    P.pff = (float)payoff_fun;
+    P.N = N;
+    P.half = N / 2;
+    P.M = M;
+    P.seed = o.seed;
+
+    const size_t store_bytes = (size_t)M * (size_t)N * rs;
+    const size_t state_bytes = (size_t)N * (sizeof(int32_t) + 2 * rs);
+    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams);
+    PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
+    PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
+    PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
+
+    const int B = o.block_size ? o.block_size : 256;
+    const int ipt = o.items_per_thread ? o.items_per_thread : 4;
+    const int grid_paths = (int)std::max<int64_t>(1, std::min<int64_t>((P.half + B - 1) / B, 1 << 20));
+    const int grid_step = (int)std::max<int64_t>(
+        1, std::min<int64_t>((N + (int64_t)B * ipt - 1) / ((int64_t)B * ipt), 1 << 20));
+    const int grid_max = std::max(grid_paths, grid_step);
+    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid_max * 10 * sizeof(pcf::CSum)));
+
+    LsmBufs b{};
+    b.store = ctx->lsm_store.ptr;
+    char* sp = ctx->lsm_state.as<char>();
+    b.when = (int32_t*)sp;
+    b.cf = sp + (size_t)N * sizeof(int32_t);
+    b.st = sp + (size_t)N * (sizeof(int32_t) + rs);
+    double* cbase = ctx->lsm_coef.as<double>();
+    b.coef = cbase;
+    b.disc = cbase + 4 * (size_t)(M + 1);
+    LsmParams* Pdev = (LsmParams*)(cbase + 5 * (size_t)(M + 1));
+    b.P = Pdev;
+    b.partials = ctx->partials.as<pcf::CSum>();
+    b.counter = ctx->counters.as<unsigned int>();
+    b.out = ctx->out.as<double>();
+
+    std::vector<double> host_blk(5 * (size_t)(M + 1), 0.0);
+    for (int k = 0; k <= M; ++k) host_blk[4 * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
+
+    if ((st = pcf_timing_begin(ctx))) return st;
+    if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
+    b.hw = src == kHost ? ctx->normals.as<double>() : nullptr;
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, host_blk.data(), host_blk.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+
+    // Captured graph per shape; re-captured if a buffer moved.
+    GraphKey key{N, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B};
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> gl(graph_mu());
+        auto& g = graph_cache()[ctx][key];
+        const bool fresh = g.exec && g.store_ptr == b.store && g.state_ptr == (void*)b.when &&
+                           g.coef_ptr == (void*)cbase && g.part_ptr == (void*)b.partials &&
+                           g.norm_ptr == (void*)b.hw;
+        if (!fresh) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = LsmGraph{};
+            hipGraph_t graph = nullptr;
+            PCF_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            if (f64) {
+                if (src == kHost) {
+                    if (ref) enqueue_lsm<double, kHost, true>(b, M, grid_paths, grid_step, B, ctx->stream);
+                    else enqueue_lsm<double, kHost, false>(b, M, grid_paths, grid_step, B, ctx->stream);
+                } else {
+                    if (ref) enqueue_lsm<double, kPhilox, true>(b, M, grid_paths, grid_step, B, ctx->stream);
+                    else enqueue_lsm<double, kPhilox, false>(b, M, grid_paths, grid_step, B, ctx->stream);
+                }
+            } else {
+                enqueue_lsm<float, kPhilox, false>(b, M, grid_paths, grid_step, B, ctx->stream);
+            }
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            if (ce != hipSuccess) return pcf_hip_fail(ctx, ce, "hipStreamEndCapture");
+            ce = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ce != hipSuccess) {
+                g.exec = nullptr;
+                return pcf_hip_fail(ctx, ce, "hipGraphInstantiate");
+            }
+            g.store_ptr = b.store;
+            g.state_ptr = (void*)b.when;
+            g.coef_ptr = (void*)cbase;
+            g.part_ptr = (void*)b.partials;
+            g.norm_ptr = (void*)b.hw;
+        }
+        exec = g.exec;
+    }
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if ((st = pcf_timing_end(ctx, 4, res))) return st;
+    const double h0 = ctx->h_out[0], h1 = ctx->h_out[1], h2 = ctx->h_out[2], h3 = ctx->h_out[3];
+
+    // Per-date flags.
+    std::vector<double> coefs(4 * (size_t)(M + 1));
+    PCF_HIP_CHECK(ctx, hipMemcpy(coefs.data(), cbase, coefs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int small = 0, fallback = 0, singular = 0;
+    for (int m = 1; m <= M - 1; ++m) {
+        const int f = (int)coefs[4 * m];
+        small += f == kFlagSmall;
+        fallback += f == kFlagReduced;
+        singular += f == kFlagSingular;
+    }
+    if (o.when_out)
+        PCF_HIP_CHECK(ctx, hipMemcpy(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost));
+
+    const double n = (double)N;
+    const double pay0 = std::max(payoff_fun * (S0 - E), 0.0);
+    // return std::max(payoff(S0,E,payoff_fun),result/(double)N)  (src/mc_amer.cpp:113)
+    res->price = std::max(pay0, h2 / n);
+    res->mean = h2 / n;
+    res->variance = N > 1 ? (h3 - h2 * (h2 / n)) / (n - 1.0) : 0.0;
+    res->std_err = std::sqrt(res->variance / n);
+    const double discT = std::exp(-r * T);
+    res->control_price = discT * h0 / n;
+    const double evar = N > 1 ? (h1 - h0 * (h0 / n)) / (n - 1.0) : 0.0;
+    res->control_se = discT * std::sqrt(evar / n);
+    res->n_paths = N;
+    res->n_steps = M;
+    res->lsm_small_itm = small;
+    res->lsm_fallback = fallback;
+    res->grid_blocks = grid_step;
+    res->block_size = B;
+    // Algorithmic HBM bytes: store written once and read back once, per-path
+    // state {cash flow, date} read + written per backward date.
+    res->bytes = n * ((double)M * rs + (double)(M - 1) * rs + 2.0 * (M - 1) * (4.0 + rs));
+    res->flops = 0.0;
+    if (singular)
+        return pcf_fail(ctx, PCF_ERR_SINGULAR, "Detereminant is not > 0 (%d dates)", singular);
+    return PCF_OK;
+}

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
@@ -83,6 +84,9 @@ struct pcf_ctx {
     double* h_out = nullptr;  // pinned mirror of `out`
 };
 
+// Releases the LSM graphs captured for a context (lsm.hip).
+void pcf_lsm_release(pcf_ctx* ctx);
+
 // Error helpers ------------------------------------------------------------
 int pcf_fail(pcf_ctx* ctx, int code, const char* fmt, ...);
 int pcf_hip_fail(pcf_ctx* ctx, hipError_t e, const char* what);
@@ -113,3 +117,11 @@ static inline pcf_opts pcf_opts_or_default(const pcf_opts* o) {
 }
 
 static inline int pcf_div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// glibc pow, never folded by the compiler.  Verification constants follow
+// the reference's -O0 build, which calls glibc pow(x, 2.0); compilers fold
+// pow(x,2) into x*x, which differs in the last bit for ~0.1% of inputs.
+static inline double libm_pow(double x, double y) {
+    double (*volatile f)(double, double) = &::pow;
+    return f(x, y);
+}

@@ -2,6 +2,8 @@
 import ctypes as C
 import os
 
+C = C  # re-exported for tests that pass pointers
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

@@ -1,0 +1,237 @@
+"""GPU parity for the basket (mc_eur_multi) and the American LSM (mc_amer).
+
+Bars as in test_gpu_parity.py: 1e-12 relative for verification / FP64
+against the oracle on identical normals; FP32 within a fraction of a
+standard error of FP64 on the same Philox paths; closed-form checks in SE.
+The LSM decision can flip for a path whose payoff lies within rounding of
+the regression value; the tests compare per-path exercise dates and require
+no flips before asserting 1e-12.
+"""
+import csv
+import math
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+REL = 1e-12
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def equicorr(d, rho):
+    return np.full((d, d), rho) + (1 - rho) * np.eye(d)
+
+
+def chol(C):
+    d = C.shape[0]
+    L = np.empty((d, d))
+    assert O.lib().orc_cholesky(d, O.ptr(np.ascontiguousarray(C)), O.ptr(L)) == 1
+    return L
+
+
+def random_corr(d, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(d, d))
+    C = A @ A.T + d * np.eye(d)
+    Dm = 1 / np.sqrt(np.diag(C))
+    return C * Dm[:, None] * Dm[None, :]
+
+
+# ---------------------------------------------------------------- basket
+@pytest.mark.parametrize("d,rho,pf", [(1, 0.0, "call"), (4, 0.5, "call"), (8, 0.3, "put"), (32, 0.3, "call"),
+                                      (13, -0.05, "call")])
+def test_basket_verification_vs_oracle(engine, d, rho, pf):
+    say("basket verify", d, rho, pf)
+    N = 20000
+    Z = O.normals_mt(100 + d, 1.0, N * d)
+    L = chol(equicorr(d, rho))
+    p = 1.0 if pf == "call" else -1.0
+    want = O.lib().orc_basket_z(100, 100, 0.1, 0.2, 1.0, N, p, d, O.ptr(L), O.ptr(Z), None, None, None)
+    got = engine.mc_eur_multi(100, 100, 0.1, 0.2, 1.0, N, pf, d, rho, normals=Z, ref_exact=True)
+    assert rel(got.price, want) < REL
+
+
+def test_basket_extension_verification_vs_oracle(engine):
+    """cfg2's extension: per-asset sigma in [0.15, 0.35] and a random SPD correlation."""
+    d, N = 8, 20000
+    rng = np.random.default_rng(2)
+    sig = rng.uniform(0.15, 0.35, size=d)
+    C = random_corr(d, 3)
+    L = chol(C)
+    Z = O.normals_mt(7, 1.0, N * d)
+    say("basket ext verify")
+    want = O.lib().orc_basket_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, d, O.ptr(L), O.ptr(Z), O.ptr(sig), None, None)
+    got = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.0, sigmas=sig, corr=C, normals=Z,
+                              ref_exact=True)
+    assert rel(got.price, want) < REL
+
+
+@pytest.mark.parametrize("d", [8, 32])
+def test_basket_fp64_philox_vs_oracle(engine, d):
+    say("basket fp64 philox", d)
+    N = 6000
+    Z = np.empty(N * d)
+    O.lib().orc_philox_basket_normals(21, N, d, O.ptr(Z))
+    L = chol(equicorr(d, 0.3))
+    want = O.lib().orc_basket_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, d, O.ptr(L), O.ptr(Z), None, None, None)
+    got = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, seed=21, precision="fp64")
+    assert rel(got.price, want) < REL
+
+
+@pytest.mark.parametrize("d,variant", [(8, 0), (32, 1), (32, 2), (16, 2), (20, 2)])
+def test_basket_fp32_vs_fp64(engine, d, variant):
+    say("basket fp32 vs fp64", d, variant)
+    N = 1 << 20
+    r64 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, seed=5, precision="fp64")
+    r32 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, seed=5, kernel_variant=variant)
+    assert abs(r32.price - r64.price) < 0.05 * r64.std_err
+
+
+def test_basket_cfg2_full_size(engine):
+    """cfg2: d = 8 and 32, 2^24 paths, sigma_i ~ U[0.15,0.35], rho = 0.3 and a random SPD matrix."""
+    N = 1 << 24
+    for d in (8, 32):
+        rng = np.random.default_rng(d)
+        sig = rng.uniform(0.15, 0.35, size=d)
+        for corr in (equicorr(d, 0.3), random_corr(d, 11)):
+            say("basket cfg2", d)
+            r64 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, sigmas=sig, corr=corr,
+                                      seed=9, precision="fp64")
+            r32 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, sigmas=sig, corr=corr, seed=9)
+            assert abs(r32.price - r64.price) < 0.05 * r64.std_err
+
+
+def test_basket_perfect_correlation_is_black_scholes(engine):
+    """rho = 1: the correlation is singular (eigen factor path, as mvnorm's
+    fallback); every asset moves together, so the basket is one GBM."""
+    say("basket rho=1")
+    r = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 20, "call", 8, 1.0, seed=4, precision="fp64")
+    assert r.lsm_fallback == 1
+    bs = O.lib().orc_bs_price(100, 100, 0.05, 0.2, 1.0, 1.0)
+    assert abs(r.price - bs) < 4 * r.std_err
+
+
+def test_basket_invalid(engine):
+    from parcompfin_amd import PcfError
+    say("basket invalid")
+    with pytest.raises(PcfError):
+        engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1000, "call", 0, 0.3)
+    with pytest.raises(PcfError):  # indefinite: rho < -1/(d-1)
+        engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1000, "call", 4, -0.5)
+    with pytest.raises(PcfError):
+        engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1000, "call", 8, 0.3, kernel_variant=2)
+
+
+# ---------------------------------------------------------------- mc_amer
+def _amer_ref(N, M, pf, w, quirk):
+    when = np.zeros(N, dtype=np.int32)
+    v = O.lib().orc_mc_amer_z(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), quirk,
+                              when.ctypes.data_as(O.C.POINTER(O.C.c_int32)))
+    return v, when
+
+
+@pytest.mark.parametrize("N,M,pf,seed", [(1024, 50, "put", 42), (16384, 50, "put", 1), (16384, 50, "call", 1),
+                                         (2048, 7, "put", 3), (4096, 2, "put", 5), (64, 1, "put", 6)])
+def test_amer_verification_vs_oracle(engine, N, M, pf, seed):
+    say("amer verify", N, M, pf, seed)
+    w = O.normals_mt(seed, math.sqrt(1.0 / M), (N // 2) * M)
+    p = 1.0 if pf == "call" else -1.0
+    want, wwhen = _amer_ref(N, M, p, w, 1)
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, pf, normals=w, ref_exact=True, return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    say("  flips", flips)
+    assert flips == 0
+    assert rel(got.price, want) < REL
+
+
+def test_amer_verification_vs_reference_fixture(engine):
+    rows = [r for r in csv.DictReader(open(os.path.join(HERE, "golden", "ref_kernels.csv")))
+            if r["prog"] == "mc_amer"]
+    for r in rows:
+        N, M, seed = int(r["N"]), int(r["M"]), int(r["seed"])
+        say("amer fixture", N, M, seed, r["payoff"])
+        T = float(r["T"])
+        w = O.normals_mt(seed, math.sqrt(T / M), (N // 2) * M)
+        got = engine.mc_amer(float(r["S0"]), float(r["E"]), float(r["r"]), float(r["sigma"]), T, N, M,
+                             r["payoff"], normals=w, ref_exact=True)
+        assert rel(got.price, float(r["result"])) < REL
+
+
+def test_amer_production_arithmetic_on_host_normals(engine):
+    """Corrected exercise test with the production basis S/E and Cholesky
+    solve vs the oracle's corrected reference basis: the fitted values agree
+    in exact arithmetic, so with no flipped decision the prices agree."""
+    N, M = 8192, 50
+    w = O.normals_mt(77, math.sqrt(1.0 / M), (N // 2) * M)
+    want, wwhen = _amer_ref(N, M, -1.0, w, 0)
+    say("amer production arithmetic, host normals")
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", normals=w, return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    say("  flips", flips)
+    assert flips <= 2
+    if flips == 0:
+        assert rel(got.price, want) < 1e-11
+    else:
+        assert rel(got.price, want) < 1e-4
+
+
+def test_amer_fp64_philox_vs_oracle(engine):
+    N, M = 8192, 50
+    w = np.empty((N // 2) * M)
+    O.lib().orc_philox_amer_normals(13, N, M, 1.0, O.ptr(w))
+    want, wwhen = _amer_ref(N, M, -1.0, w, 0)
+    say("amer fp64 philox vs oracle")
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=13, precision="fp64", return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    assert flips <= 2
+    assert rel(got.price, want) < (1e-11 if flips == 0 else 1e-4)
+
+
+def test_amer_cfg4_fp32_vs_fp64_and_bounds(engine):
+    N, M = 1 << 20, 50
+    say("amer cfg4 fp32/fp64")
+    r64 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=17, precision="fp64")
+    r32 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=17)
+    euro = O.lib().orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)
+    lattice = 5.798877611783797  # reference binom_vanilla_amer, N = 2000 (tests/golden)
+    say(f"  fp64 {r64.price:.6f} fp32 {r32.price:.6f} se {r64.std_err:.2e} euro-ctl {r64.control_price:.6f}")
+    assert abs(r32.price - r64.price) < r64.std_err
+    assert r64.price > euro
+    assert abs(r64.control_price - euro) < 4 * r64.control_se
+    assert abs(r64.price - lattice) < 0.05  # LSM with a quadratic basis sits slightly below
+
+
+def test_amer_graph_replay_is_deterministic(engine):
+    say("amer graph replay")
+    a = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 16, 50, "put", seed=3)
+    b = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 16, 50, "put", seed=3)
+    c = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 16, 50, "put", seed=4)
+    d = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 16, 50, "put", seed=3)
+    assert a.price == b.price == d.price and c.price != a.price
+
+
+def test_amer_errors_like_reference(engine):
+    from parcompfin_amd import PcfError
+    say("amer errors")
+    with pytest.raises(PcfError):  # odd N: include/common.h:180
+        engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1001, 50, "put")
+    # sigma = 0: every in-the-money x is equal, the Gram matrix is singular and
+    # the reference throws "Detereminant is not > 0".
+    w = np.zeros((1024 // 2) * 10)
+    with pytest.raises(PcfError) as ei:
+        engine.mc_amer(100, 110, 0.01, 0.0, 1.0, 1024, 10, "put", normals=w, ref_exact=True)
+    assert ei.value.status == 5
+    # production falls back to a reduced basis instead
+    r = engine.mc_amer(100, 110, 0.01, 0.0, 1.0, 1024, 10, "put", normals=w)
+    assert r.lsm_fallback > 0
