@@ -1,24 +1,33 @@
 #!/usr/bin/env python3
-"""bench.py -- throughput of the pricing hot path on one MI355X.
+"""bench.py -- the pricing hot path served on one MI355X.
 
 Prints ONE JSON line on stdout (diagnostics go to stderr).
 
-Workload (config.workload): the fixed-strike arithmetic Asian call of the
-configuration S0=K=100, r=0.05, sigma=0.25, T=1, M=252 monitoring dates,
-N=2^22 paths -- the largest single-GPU path-step configuration.  A step is one
-full pricing (path generation + payoff reduction + FP64 mean/variance) in the
-production FP32/Kahan form, inputs resident, synchronous.  value is
-paths*steps/s over the K timed steps.  The same K steps are repeated in the
-plain FP64 form for the FP32-vs-FP64 ratio and the error in standard errors.
+Workload (config.workload = "served mix"): a seeded request set mixing the
+single-GPU configurations -- European call (S0=K=100, r=0.05, sigma=0.2),
+basket call on 8 equicorrelated assets (rho=0.3, sigma_i ~ U[0.15,0.35]),
+arithmetic Asian call (sigma=0.25, M=252) and American put by Longstaff-
+Schwartz (r=0.06, sigma=0.2, M=50 exercise dates, path store on the device)
+-- with path counts log-uniform in [2^14, 2^22].  One step submits the whole
+set to the continuous-batching server (pcf_server_*) and waits for every
+result; value = sum over requests of paths x steps (1 for single-step
+products, M otherwise) / step time, in the production FP32/Kahan form.  The
+same steps in the plain FP64 form give the FP32-vs-FP64 ratio.  A second
+phase feeds a Poisson stream at ~75% of the measured capacity and reports
+p50/p99 latency overall and for 2^16-path requests.
 
-roofline: the dominant kernel (k_asia) -- 14 algorithmic FP32 flops per
-path-step (normal 6, I update 4, S update 4) x N x M per launch, divided by
-its average launch time from HIP events recorded by the library on the stream
-it launches on; peak = MI355X FP32 vector 157.3 TFLOP/s.
+Every configuration is also priced at its full size in FP32/Kahan and FP64
+with the error in standard errors and the closed-form checks.
 
-cpu_baseline: the reference's own OpenMP program src/mc_asia_omp.cpp built
-from source (oracle/_ref/mc_asia_omp, -O2 -fopenmp) on the host cores, on a
-bounded sample of the same workload; falls back to the oracle's OpenMP port.
+roofline: the dominant kernel of the served mix by device time, measured
+at its full configuration with HIP events recorded by the library on the
+stream it launches on.
+
+cpu_baseline: the reference's own OpenMP programs (src/mc_eur_omp.cpp,
+src/mc_asia_omp.cpp, src/mc_amer_omp.cpp built from source at -O2 -fopenmp
+into oracle/_ref) on a bounded sample of the same mix (the reference basket
+program needs Eigen/Boost, absent here, so basket requests are left out of
+the sample); paths x steps / total T_calculation.
 """
 import argparse
 import json
@@ -31,9 +40,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3
-PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
+PEAK = {"fp32": 157.3, "fp64": 78.6, "hbm": 8000.0}
 
 
 def log(*a):
@@ -58,116 +65,273 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(N_full, M, budget_s):
-    """Reference mc_asia_omp on the host cores, on a bounded sample of N."""
-    threads = max(1, min(16, os.cpu_count() or 1))
-    exe = os.path.join(ROOT, "oracle", "_ref", "mc_asia_omp")
-    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
-    if os.path.exists(exe):
-        def run(n):
-            out = subprocess.run([exe, "call", "100", "100", "0.05", "0.25", "1", str(n), str(M), str(threads)],
-                                 env=env, check=True, capture_output=True, text=True, timeout=600).stdout
-            return float(out.strip().split(",")[12])  # T_calculation
-        kind = "reference"
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import _oracle as O
-
-        def run(n):
-            t0 = time.perf_counter()
-            O.lib().orc_mc_asia_omp(100, 100, 0.05, 0.25, 1.0, n, M, 1.0, threads, 7)
-            return time.perf_counter() - t0
-        kind = "port"
-    n0 = 1 << 14
-    t0 = max(run(n0), 1e-6)
-    n = int(min(N_full, max(n0, n0 * budget_s / t0)))
-    n = max(1024, (n // 1024) * 1024)
-    best = None
-    for _ in range(2 if n * M / (n0 * M / t0) < 0.5 * budget_s else 1):
-        t = run(n)
-        best = t if best is None else min(best, t)
-    return {
-        "value": n * M / best,
-        "unit": "paths·steps/s",
-        "cores": threads,
-        "kind": kind,
-        "sample": f"mc_asia call S0=K=100 r=0.05 sigma=0.25 T=1 M={M}, N={n} paths "
-                  f"({n / N_full:.3f} of the {N_full}-path workload), FP64, "
-                  f"{'src/mc_asia_omp.cpp built -O2 -fopenmp' if kind == 'reference' else 'oracle OpenMP port'}, "
-                  f"OMP_PROC_BIND=close, {threads} threads on '{cpu_model()}', T_calculation {best:.3f} s",
-    }
+# ---------------------------------------------------------------- the mix
+def make_mix(seed, n, lo=14.0, hi=22.0):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    kinds = ["eur", "basket", "asia", "amer"]
+    sig8 = np.random.default_rng(8).uniform(0.15, 0.35, size=8)
+    mix = []
+    for i in range(n):
+        k = kinds[int(rng.integers(0, 4))]
+        N = max(2, int(round(2 ** rng.uniform(lo, hi))) // 2 * 2)
+        s = int(rng.integers(1, 2**62))
+        if k == "eur":
+            mix.append(dict(kind="eur", args=(100.0, 100.0, 0.05, 0.2, 1.0, N, "call"), kw={}, steps=1, seed=s))
+        elif k == "basket":
+            mix.append(dict(kind="basket", args=(100.0, 100.0, 0.05, 0.2, 1.0, N, "call"),
+                            kw=dict(assets=8, rho=0.3, sigmas=sig8), steps=1, seed=s))
+        elif k == "asia":
+            mix.append(dict(kind="asia", args=(100.0, 100.0, 0.05, 0.25, 1.0, N, "call"), kw=dict(M=252),
+                            steps=252, seed=s))
+        else:
+            mix.append(dict(kind="amer", args=(100.0, 100.0, 0.06, 0.2, 1.0, N, "put"), kw=dict(M=50),
+                            steps=50, seed=s))
+    return mix
 
 
-def time_steps(fn, K, W):
-    for _ in range(W):
-        fn(0)
-    from parcompfin_amd import Engine  # noqa: F401
-    res = []
+def path_steps(mix):
+    return sum(q["args"][5] * q["steps"] for q in mix)
+
+
+def serve_once(srv, mix, precision):
     t0 = time.perf_counter()
-    for k in range(K):
-        res.append(fn(k + 1))
-    t1 = time.perf_counter()
-    return (t1 - t0), res
+    tickets = [srv.submit(q["kind"], *q["args"], seed=q["seed"], precision=precision, **q["kw"]) for q in mix]
+    res = [srv.wait(t) for t in tickets]
+    return time.perf_counter() - t0, res
+
+
+def poisson_phase(srv, mix, rate, precision="fp32k"):
+    """Open-loop arrivals at `rate` req/s from a submitter thread; returns latencies."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(123)
+    gaps = rng.exponential(1.0 / rate, size=len(mix))
+    tickets = [None] * len(mix)
+    t_start = time.perf_counter()
+
+    def submitter():
+        t = time.perf_counter()
+        for i, q in enumerate(mix):
+            t += gaps[i]
+            while True:
+                d = t - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.002))
+            tickets[i] = srv.submit(q["kind"], *q["args"], seed=q["seed"], precision=precision, **q["kw"])
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    res = [None] * len(mix)
+    for i in range(len(mix)):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        res[i] = srv.wait(tickets[i])
+    th.join()
+    return time.perf_counter() - t_start, res
+
+
+def pct(v, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), p)) if len(v) else None
+
+
+# ---------------------------------------------------------------- per config
+def per_config(eng):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    out = {}
+
+    def pair(fn):
+        fn("fp32k")
+        fn("fp64")  # warm both
+        r32 = fn("fp32k")
+        r64 = fn("fp64")
+        return r32, r64
+
+    def summary(name, r32, r64, steps, check=None):
+        d = {"price_fp32k": r32.price, "price_fp64": r64.price, "std_err": r64.std_err,
+             "fp32k_minus_fp64_in_se": (r32.price - r64.price) / r64.std_err if r64.std_err else None,
+             "kernel_ms_fp32k": r32.kernel_ms, "kernel_ms_fp64": r64.kernel_ms,
+             "paths_steps_per_s_fp32k": r32.n_paths * steps / (r32.kernel_ms * 1e-3),
+             "paths_steps_per_s_fp64": r64.n_paths * steps / (r64.kernel_ms * 1e-3),
+             "fp32k_over_fp64": r64.kernel_ms / r32.kernel_ms}
+        if check:
+            d.update(check)
+        out[name] = d
+        log(name, json.dumps({k: (round(v, 6) if isinstance(v, float) else v) for k, v in d.items()}))
+
+    bs = 10.450583572185565
+    r32, r64 = pair(lambda p: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=1, precision=p))
+    summary("cfg0_mc_eur_1e6", r32, r64, 1, {"black_scholes": bs, "fp64_minus_bs_in_se": (r64.price - bs) / r64.std_err})
+    for N in (10**5, 10**6):
+        eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
+        rc = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
+        rl = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
+        out[f"cfg1_binom_embar_N{N}"] = {"price_chain": rc.price, "kernel_ms_chain": rc.kernel_ms,
+                                         "price_lgamma": rl.price, "kernel_ms_lgamma": rl.kernel_ms,
+                                         "lgamma_minus_chain_rel": (rl.price - rc.price) / rc.price,
+                                         "log_adds_per_s_chain": rc.flops / (rc.kernel_ms * 1e-3)}
+        log(f"cfg1 N={N}", out[f"cfg1_binom_embar_N{N}"])
+    for d in (8, 32):
+        sig = np.random.default_rng(d).uniform(0.15, 0.35, size=d)
+        r32, r64 = pair(lambda p: eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3,
+                                                   sigmas=sig, seed=2, precision=p))
+        summary(f"cfg2_basket_d{d}_2^24", r32, r64, 1, {"tflops_fp32k": r32.flops / (r32.kernel_ms * 1e-3) / 1e12})
+    r32, r64 = pair(lambda p: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=3, precision=p,
+                                          control=True))
+    geo = 6.551716578
+    summary("cfg3_mc_asia_2^22x252", r32, r64, 252,
+            {"geometric_control_fp32k": r32.control_price, "geometric_closed_form": geo,
+             "control_minus_closed_in_se": (r32.control_price - geo) / r32.control_se})
+    r32, r64 = pair(lambda p: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=4, precision=p))
+    summary("cfg4_mc_amer_put_2^20x50", r32, r64, 50,
+            {"european_put": 5.166002511, "european_control_fp32k": r32.control_price,
+             "lattice_binom_vanilla_amer_N2000": 5.798877611783797,
+             "hbm_gbs_fp32k": r32.bytes / (r32.kernel_ms * 1e-3) / 1e9})
+    return out
+
+
+def asia_roofline(eng, K=10):
+    """k_asia<fp32k> at 2^22 x 252: algorithmic FP32 flops / HIP-event time."""
+    N, M = 1 << 22, 252
+    rs = [eng.mc_asia(100, 100, 0.05, 0.25, 1.0, N, M, "call", seed=100 + k) for k in range(K + 1)][1:]
+    kms = sum(r.kernel_ms for r in rs) / K
+    fl = rs[0].flops
+    ach = fl / (kms * 1e-3) / 1e12
+    return {"bound": "mfma",
+            "pipe": "VALU FP32 (non-packed FMA + transcendental units); 157.3 TFLOP/s is MI355X's FP32 peak",
+            "kernel": "k_asia<fp32k> (mc_asia, 2^22 paths x 252 steps)", "achieved": ach,
+            "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": None,
+            "kernel_ms": kms, "flops_per_launch": fl,
+            "flops_per_unit": "14 per path-step (normal 6, I update 4, S update 4)"}
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline(mix, budget_s):
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
+    exes = {"eur": "mc_eur_omp", "asia": "mc_asia_omp", "amer": "mc_amer_omp"}
+    if not all(os.path.exists(os.path.join(ref, e)) for e in exes.values()):
+        return {"value": None, "unit": "paths·steps/s", "cores": threads, "kind": None,
+                "sample": "oracle/_ref reference programs not built"}
+    sample = [q for q in mix if q["kind"] in exes]
+    # scale every request by the same factor so the whole sample is ~budget_s
+    def run(q, N):
+        a = q["args"]
+        argv = [os.path.join(ref, exes[q["kind"]]), a[6]] + [str(x) for x in a[:5]] + [str(N)]
+        if q["kind"] != "eur":
+            argv.append(str(q["kw"]["M"]))
+        argv.append(str(threads))
+        out = subprocess.run(argv, env=env, check=True, capture_output=True, text=True, timeout=600).stdout
+        return float(out.strip().split(",")[12])
+    probe = 1 << 12
+    t_probe = sum(run(q, probe) for q in sample[:8])
+    ps_probe = sum(probe * q["steps"] for q in sample[:8])
+    rate0 = ps_probe / max(t_probe, 1e-6)
+    total_ps = path_steps(sample)
+    scale = min(1.0, budget_s * rate0 / total_ps)
+    tot_t, tot_ps = 0.0, 0
+    for q in sample:
+        N = max(64, int(q["args"][5] * scale) // 2 * 2)
+        tot_t += run(q, N)
+        tot_ps += N * q["steps"]
+    return {"value": tot_ps / tot_t, "unit": "paths·steps/s", "cores": threads, "kind": "reference",
+            "sample": f"{len(sample)} European/Asian/American requests of the served mix with path counts "
+                      f"scaled by {scale:.4f} ({tot_ps:.3g} paths·steps), run by the reference's "
+                      f"mc_eur_omp/mc_asia_omp/mc_amer_omp (built -O2 -fopenmp from src/), FP64, "
+                      f"{threads} threads, OMP_PROC_BIND=close, on '{cpu_model()}'; sum of T_calculation "
+                      f"{tot_t:.2f} s; basket requests excluded (reference needs Eigen/Boost)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--latency-requests", type=int, default=400)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", type=int, default=1 << 22)
-    ap.add_argument("--dates", type=int, default=252)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU only: every box here has one GPU and no multi-GPU path is "
-            "run in this build (sharding is described in DESIGN.md). Use --gpus 1.")
+        log("bench.py measures one GPU: every box here has one GPU and no multi-GPU path runs in this "
+            "build (how the path would shard is described in DESIGN.md). Use --gpus 1.")
         sys.exit(2)
     K, W = args.steps, args.warmup
-    N, M = args.paths, args.dates
 
-    from parcompfin_amd import Engine
+    from parcompfin_amd import Engine, Server, autotune
     eng = Engine(0)
     info = eng.device_info()
     log("device:", info)
+    tuned = []
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = autotune(eng, max_log2_paths=22)
+        log(f"autotune: {len(tuned)} entries in {time.perf_counter() - t0:.1f} s")
 
-    S0, E, r, sig, T = 100.0, 100.0, 0.05, 0.25, 1.0
+    configs = {} if args.no_configs else per_config(eng)
+    roof = asia_roofline(eng)
+    log("roofline:", roof)
 
-    def step32(k):
-        return eng.mc_asia(S0, E, r, sig, T, N, M, "call", seed=1000 + k, precision="fp32k")
+    mix = make_mix(2026, args.requests)
+    ps = path_steps(mix)
+    srv = Server(eng, mem_budget_bytes=16 << 30, max_inflight=args.slots)
+    results = {}
+    for prec in ("fp32k", "fp64"):
+        for _ in range(W):
+            serve_once(srv, mix, prec)
+        ts, rs = [], None
+        for _ in range(K):
+            t, rs = serve_once(srv, mix, prec)
+            ts.append(t)
+        results[prec] = (ts, rs)
+        log(f"mix {prec}: {1e3 * sum(ts) / K:.2f} ms/step, {ps * K / sum(ts):.4g} paths·steps/s")
+    t32, r32 = results["fp32k"]
+    t64, r64 = results["fp64"]
+    value = ps * K / sum(t32)
+    value64 = ps * K / sum(t64)
+    dse = [abs(a.price - b.price) / b.std_err for a, b in zip(r32, r64) if b.std_err > 0]
+    by_kind = {}
+    for q, a in zip(mix, r32):
+        k = by_kind.setdefault(q["kind"], {"requests": 0, "paths_steps": 0, "device_ms": 0.0})
+        k["requests"] += 1
+        k["paths_steps"] += q["args"][5] * q["steps"]
+        k["device_ms"] += a.kernel_ms / max(a.batch_size, 1)
 
-    def step64(k):
-        return eng.mc_asia(S0, E, r, sig, T, N, M, "call", seed=1000 + k, precision="fp64")
-
-    eng.synchronize()
-    t32, res32 = time_steps(step32, K, W)
-    eng.synchronize()
-    t64, res64 = time_steps(step64, K, W)
-    eng.synchronize()
-
-    ms32 = 1e3 * t32 / K
-    ms64 = 1e3 * t64 / K
-    value = N * M / (t32 / K)
-    value64 = N * M / (t64 / K)
-    kms = sum(x.kernel_ms for x in res32) / K
-    kms64 = sum(x.kernel_ms for x in res64) / K
-    flops = res32[0].flops
-    achieved = flops / (kms * 1e-3) / 1e12
-    errs_se = [abs(a.price - b.price) / b.std_err for a, b in zip(res32, res64)]
-    log(f"fp32k {ms32:.3f} ms/step (kernel {kms:.3f} ms)  fp64 {ms64:.3f} ms/step (kernel {kms64:.3f} ms)")
-    log(f"price fp32k {res32[0].price:.8f} fp64 {res64[0].price:.8f} se {res64[0].std_err:.2e} "
-        f"max |d|/se {max(errs_se):.2e}")
+    # Poisson latency phase at ~75% of capacity.
+    req_rate = len(mix) * K / sum(t32)
+    lat_mix = make_mix(77, args.latency_requests)
+    for q in lat_mix[: len(lat_mix) // 8]:  # a share of exactly-2^16 requests
+        q["args"] = q["args"][:5] + (1 << 16,) + q["args"][6:]
+    rate = 0.75 * req_rate * (ps / len(mix)) / (path_steps(lat_mix) / len(lat_mix))
+    wall, lres = poisson_phase(srv, lat_mix, rate)
+    lat = [r.extra["timing"]["latency_ms"] for r in lres]
+    lat16 = [r.extra["timing"]["latency_ms"] for q, r in zip(lat_mix, lres) if q["args"][5] == 1 << 16]
+    lat_by = {}
+    for q, r in zip(lat_mix, lres):
+        lat_by.setdefault(q["kind"], []).append(r.extra["timing"]["latency_ms"])
+    latency = {"arrival_rate_req_per_s": rate, "requests": len(lat_mix), "wall_s": wall,
+               "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+               "p50_ms_2^16_paths": pct(lat16, 50), "p99_ms_2^16_paths": pct(lat16, 99), "n_2^16": len(lat16),
+               "p99_ms_by_kind": {k: pct(v, 99) for k, v in lat_by.items()},
+               "served_paths_steps_per_s": path_steps(lat_mix) / wall}
+    log("latency:", latency)
+    srv.close()
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(N, M, args.cpu_seconds)
+            cpu = cpu_baseline(mix, args.cpu_seconds)
             log("cpu baseline:", cpu)
-        except Exception as e:  # reported, never silently replaced
+        except Exception as e:  # reported, never replaced by something else
             log("cpu baseline failed:", repr(e))
-            cpu = {"value": None, "unit": "paths·steps/s", "cores": None, "kind": None,
-                   "sample": f"failed: {e!r}"}
+            cpu = {"value": None, "unit": "paths·steps/s", "cores": None, "kind": None, "sample": f"failed: {e!r}"}
 
     line = {
         "metric": metric_name(),
@@ -176,24 +340,25 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": ms32,
+        "ms_per_step": 1e3 * sum(t32) / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded Philox4x32-10 normals generated on the device)",
-        "config": {"workload": f"mc_asia fixed-strike arithmetic Asian call S0=K=100 r=0.05 sigma=0.25 "
-                               f"T=1 M={M} N={N}, FP32 paths + Kahan/FP64 sums",
-                   "paths": N, "steps_per_path": M, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "pipe": "VALU FP32 (vector): 157.3 TFLOP/s is the FP32 peak of both the vector (packed) and matrix pipes", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_asia<fp32k>", "kernel_ms": kms,
-                     "flops_per_launch": flops},
+        "data": "synthetic (seeded request mix; Philox4x32-10 normals generated on the device)",
+        "config": {"workload": "served mix: European (cfg0), basket d=8 (cfg2), Asian M=252 (cfg3), American "
+                               "put M=50 (cfg4), paths log-uniform in 2^14..2^22, FP32 paths + Kahan/FP64 sums",
+                   "requests_per_step": len(mix), "paths_steps_per_step": ps, "slots": args.slots,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "fp64": {"value": value64, "ms_per_step": ms64, "kernel_ms": kms64,
-                 "fp32k_over_fp64": value / value64},
-        "price": {"fp32k": res32[0].price, "fp64": res64[0].price, "std_err": res64[0].std_err,
-                  "max_abs_diff_in_se": max(errs_se)},
+        "fp64": {"value": value64, "ms_per_step": 1e3 * sum(t64) / K, "fp32k_over_fp64": value / value64},
+        "accuracy": {"max_abs_fp32k_minus_fp64_in_se": max(dse) if dse else None,
+                     "mean_abs_fp32k_minus_fp64_in_se": sum(dse) / len(dse) if dse else None},
+        "latency": latency,
+        "mix_by_kind": by_kind,
+        "per_config": configs,
+        "autotune": tuned,
         "device": info["name"],
     }
     print(json.dumps(line), flush=True)

@@ -107,7 +107,8 @@ typedef struct pcf_result {
                               eigen-decomposition (not positive definite) */
     int32_t grid_blocks;   /* launch shape of the dominant kernel */
     int32_t block_size;
-    int32_t reserved[4];
+    int32_t batch_size;    /* requests priced by the same launch (1 when alone) */
+    int32_t reserved[3];
 } pcf_result;
 
 typedef struct pcf_ctx pcf_ctx;
@@ -123,7 +124,8 @@ int pcf_device_info(pcf_ctx* ctx, char* name, int name_len, int* compute_units,
 void pcf_opts_init(pcf_opts* opts);
 int pcf_abi_version(void);
 /* sizeof of an ABI record by name ("pcf_opts", "pcf_result", "pcf_basket_ext",
- * "pcf_request", "pcf_server_config", "pcf_request_timing"); -1 if unknown.
+ * "pcf_request", "pcf_server_config", "pcf_request_timing", "pcf_tune_entry");
+ * -1 if unknown.
  * Lets FFI bindings verify their struct mirrors without a device. */
 int pcf_struct_size(const char* name);
 
@@ -176,6 +178,27 @@ int pcf_mc_eur_multi(pcf_ctx* ctx, double S0, double E, double r, double sigma, 
                      int64_t N, double payoff_fun, int32_t assets, double rho,
                      const pcf_basket_ext* ext, const pcf_opts* opts, pcf_result* res);
 
+/* ---- autotuner --------------------------------------------------------- */
+/* One tuned launch shape: threads per block and work items per thread for a
+ * (kind, arithmetic mode, size class = ceil(log2 N)) triple, with the best
+ * measured kernel time. */
+typedef struct pcf_tune_entry {
+    int32_t kind, mode, size_class, block_size, items_per_thread, candidates;
+    double best_ms, default_ms;
+} pcf_tune_entry;
+
+/* Sweeps block size and items per thread for the Philox FP32/Kahan forms of
+ * the kinds in kind_mask (bit k = enum pcf_kind k; EUR, ASIA, BASKET, AMER) at
+ * size classes 2^14 .. 2^max_log2_paths, times each candidate on the device,
+ * and installs the fastest shape in the process-wide table every context
+ * (and the server) consults when opts leave block_size / items_per_thread 0.
+ * A shape change moves the reduction tree, so results change in the last
+ * bits.  Fills up to max_out entries; *n_out receives the count. */
+int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_paths, pcf_tune_entry* out,
+                 int32_t max_out, int32_t* n_out);
+/* Forgets every tuned shape (back to the built-in defaults). */
+void pcf_tune_reset(void);
+
 /* Standard normals of the device generator: out[p*K + k] = normal k of path
  * path0+p of `stream` (the exact values the pricing kernels consume, in FP64
  * or rounded to FP32 per `precision`).  For reproducibility checks. */
@@ -194,6 +217,8 @@ typedef struct pcf_request {
     double S0, E, r, sigma, T, payoff_fun, rho;
     int64_t N;
     pcf_opts opts;       /* seed, precision; rng must be PCF_RNG_PHILOX */
+    const pcf_basket_ext* basket; /* basket extension or NULL; must stay valid
+                                     until pcf_server_wait returns */
 } pcf_request;
 
 typedef struct pcf_server_config {
@@ -219,7 +244,10 @@ int pcf_server_submit(pcf_server* srv, const pcf_request* req, uint64_t* ticket)
 /* Block until the ticket's request completes; the ticket is then released. */
 int pcf_server_wait(pcf_server* srv, uint64_t ticket, pcf_result* res,
                     pcf_request_timing* timing);
+/* Drains queued requests, joins the slots and frees everything. */
 int pcf_server_destroy(pcf_server* srv);
+/* Message of the last refused submit or failed launch. */
+const char* pcf_server_last_error(pcf_server* srv);
 
 #ifdef __cplusplus
 }

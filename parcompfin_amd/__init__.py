@@ -6,6 +6,7 @@ include/pcf.h).  This package holds its sources (csrc/), the drop-in CLIs
 function names.
 """
 from ._native import LIB_PATH, PcfError, load  # noqa: F401
-from .api import Engine, Result, payoff_code  # noqa: F401
+from .api import Engine, Result, Server, autotune, payoff_code, tune_reset  # noqa: F401
 
-__all__ = ["Engine", "Result", "PcfError", "load", "LIB_PATH", "payoff_code"]
+__all__ = ["Engine", "Result", "Server", "PcfError", "load", "LIB_PATH", "payoff_code", "autotune",
+           "tune_reset"]

@@ -81,7 +81,8 @@ class PcfResult(C.Structure):
         ("lsm_fallback", C.c_int32),
         ("grid_blocks", C.c_int32),
         ("block_size", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("batch_size", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def as_dict(self) -> dict:
@@ -112,6 +113,20 @@ class PcfRequest(C.Structure):
         ("rho", C.c_double),
         ("N", C.c_int64),
         ("opts", PcfOpts),
+        ("basket", C.POINTER(PcfBasketExt)),
+    ]
+
+
+class PcfTuneEntry(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("mode", C.c_int32),
+        ("size_class", C.c_int32),
+        ("block_size", C.c_int32),
+        ("items_per_thread", C.c_int32),
+        ("candidates", C.c_int32),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
     ]
 
 
@@ -159,6 +174,9 @@ SIGNATURES = {
     "pcf_server_submit": (C.c_int, [C.c_void_p, C.POINTER(PcfRequest), C.POINTER(C.c_uint64)]),
     "pcf_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(PcfResult), C.POINTER(PcfRequestTiming)]),
     "pcf_server_destroy": (C.c_int, [C.c_void_p]),
+    "pcf_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "pcf_autotune": (C.c_int, [_ctx, _i32, _i32, C.POINTER(PcfTuneEntry), _i32, C.POINTER(_i32)]),
+    "pcf_tune_reset": (None, []),
 }
 
 STRUCTS = {
@@ -168,6 +186,7 @@ STRUCTS = {
     "pcf_request": PcfRequest,
     "pcf_server_config": PcfServerConfig,
     "pcf_request_timing": PcfRequestTiming,
+    "pcf_tune_entry": PcfTuneEntry,
 }
 
 _lib = None

@@ -57,6 +57,7 @@ class Result:
     lsm_fallback: int
     grid_blocks: int
     block_size: int
+    batch_size: int
     extra: dict = field(default_factory=dict)
 
     @classmethod
@@ -202,3 +203,96 @@ class Engine:
                                   prec, out.ctypes.data_as(C.POINTER(C.c_double)))
         self._check(st)
         return out
+
+
+KINDS = {"eur": N_.PCF_KIND_EUR, "asia": N_.PCF_KIND_ASIA, "amer": N_.PCF_KIND_AMER,
+         "basket": N_.PCF_KIND_BASKET, "binom": N_.PCF_KIND_BINOM}
+
+
+def autotune(engine: "Engine", kinds=("eur", "asia", "basket", "amer"), max_log2_paths=22) -> list:
+    """Runs the launch-shape autotuner (process-wide table) and returns its entries."""
+    mask = 0
+    for k in kinds:
+        mask |= 1 << KINDS[k]
+    out = (N_.PcfTuneEntry * 128)()
+    n = C.c_int32()
+    engine._check(engine.lib.pcf_autotune(engine.ctx, mask, int(max_log2_paths), out, 128, C.byref(n)))
+    return [{f: getattr(out[i], f) for f, _ in N_.PcfTuneEntry._fields_} for i in range(n.value)]
+
+
+def tune_reset():
+    N_.load().pcf_tune_reset()
+
+
+class Server:
+    """Continuous-batching request loop (libpcf pcf_server_*)."""
+
+    def __init__(self, engine: Engine, mem_budget_bytes: int = 0, max_inflight: int = 0, use_graphs: int = 1):
+        self.lib = engine.lib
+        cfg = N_.PcfServerConfig()
+        cfg.mem_budget_bytes = int(mem_budget_bytes)
+        cfg.max_inflight = int(max_inflight)
+        cfg.use_graphs = int(use_graphs)
+        h = C.c_void_p()
+        st = self.lib.pcf_server_create(engine.ctx, C.byref(cfg), C.byref(h))
+        if st != N_.PCF_OK:
+            raise N_.PcfError(st, "pcf_server_create failed")
+        self.srv = h
+        self._keep = {}
+
+    def last_error(self) -> str:
+        return (self.lib.pcf_server_last_error(self.srv) or b"").decode()
+
+    def submit(self, kind, S0, E, r, sigma, T, N, payoff_fun, M=0, assets=1, rho=0.0, sigmas=None,
+               spots=None, weights=None, corr=None, **opts) -> int:
+        q = N_.PcfRequest()
+        q.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
+        q.M, q.assets = int(M), int(assets)
+        q.S0, q.E, q.r, q.sigma, q.T = S0, E, r, sigma, T
+        q.payoff_fun = payoff_code(payoff_fun)
+        q.rho = rho
+        q.N = int(N)
+        o, keep = Engine.make_opts(**opts)
+        q.opts = o
+        if any(v is not None for v in (sigmas, spots, weights, corr)):
+            ext = N_.PcfBasketExt()
+            for name, v in (("sigma", sigmas), ("S0", spots), ("weights", weights), ("corr", corr)):
+                if v is not None:
+                    a = np.ascontiguousarray(v, dtype=np.float64).ravel()
+                    keep.append(a)
+                    setattr(ext, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+            keep.append(ext)
+            q.basket = C.pointer(ext)
+        t = C.c_uint64()
+        st = self.lib.pcf_server_submit(self.srv, C.byref(q), C.byref(t))
+        if st != N_.PCF_OK:
+            raise N_.PcfError(st, self.last_error())
+        self._keep[t.value] = keep
+        return t.value
+
+    def wait(self, ticket: int):
+        res = N_.PcfResult()
+        tm = N_.PcfRequestTiming()
+        st = self.lib.pcf_server_wait(self.srv, int(ticket), C.byref(res), C.byref(tm))
+        self._keep.pop(int(ticket), None)
+        if st != N_.PCF_OK:
+            raise N_.PcfError(st, self.last_error())
+        timing = {f: getattr(tm, f) for f, _ in N_.PcfRequestTiming._fields_}
+        return Result.from_c(res, timing=timing)
+
+    def close(self):
+        if getattr(self, "srv", None):
+            self.lib.pcf_server_destroy(self.srv)
+            self.srv = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

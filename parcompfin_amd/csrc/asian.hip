@@ -5,7 +5,8 @@
 // running integral: it includes S_0 and excludes S_M; reproduced as written.)
 //
 // One path per lane, paths contiguous across lanes; one Philox call feeds 4
-// steps.  FP32 form: S and the exponent in FP32 on the hardware exp2 unit, I
+// steps.  One kernel serves one request or a batch (see eur.hip: each job
+// reduces over exactly the blocks a lone launch of it would use).  FP32 form: S and the exponent in FP32 on the hardware exp2 unit, I
 // Kahan-compensated in FP32, per-path payoffs summed in FP64.  The optional
 // geometric control averages log S over t_1..t_M on the same paths.
 #include "pcf_internal.hpp"
@@ -112,16 +113,25 @@ __device__ __forceinline__ double asia_path_ref(const AsiaArgs& a, const double*
     return v > 0.0 ? v : 0.0;
 }
 
+struct AsiaJob {
+    AsiaArgs a;
+    const double* hz;
+    pcf::CSum* partials;
+    unsigned int* counter;
+    double* out;
+    unsigned int first_block, nblk;
+};
+static_assert(sizeof(AsiaJob) <= kJobBytes, "job record exceeds its staging slot");
+
 template <int MODE, bool CONTROL>
-__global__ __launch_bounds__(1024) void k_asia(AsiaArgs a, const double* __restrict__ hz,
-                                               pcf::CSum* __restrict__ partials,
-                                               unsigned int* __restrict__ counter,
-                                               double* __restrict__ out) {
+__device__ __forceinline__ void asia_body(const AsiaArgs& a, const double* __restrict__ hz,
+                                          pcf::CSum* __restrict__ partials,
+                                          unsigned int* __restrict__ counter, double* __restrict__ out,
+                                          unsigned int blk, unsigned int nblk, pcf::CSum* lds) {
     constexpr int K = CONTROL ? 4 : 2;
-    __shared__ pcf::CSum lds[16 * K];
     double s = 0.0, s2 = 0.0, g = 0.0, g2 = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.N; p += stride) {
+    const int64_t stride = (int64_t)nblk * blockDim.x;
+    for (int64_t p = (int64_t)blk * blockDim.x + threadIdx.x; p < a.N; p += stride) {
         double v, gv = 0.0;
         if (MODE == kFp32k) {
             float pv, pg = 0.f;
@@ -147,112 +157,197 @@ __global__ __launch_bounds__(1024) void k_asia(AsiaArgs a, const double* __restr
         acc.v[2 % K] = {g, 0.0};
         acc.v[3 % K] = {g2, 0.0};
     }
-    if (pcf::grid_reduce<K>(acc, partials, counter, lds, tot) && threadIdx.x == 0) {
-        const double n = (double)a.N;
-#pragma unroll
-        for (int k = 0; k < K; k += 2) {
-            const double sum = tot.v[k].s + tot.v[k].c;
-            const double sq = tot.v[k + 1].s + tot.v[k + 1].c;
-            const double mean = sum / n;
-            out[2 * k + 0] = sum;
-            out[2 * k + 1] = sq;
-            out[2 * k + 2] = mean;
-            out[2 * k + 3] = a.N > 1 ? (sq - sum * mean) / (n - 1.0) : 0.0;
-        }
-    }
+    if (pcf::grid_reduce<K>(acc, partials, counter, lds, tot, blk, nblk) && threadIdx.x == 0)
+        pcf::write_moments<K / 2>(tot, a.N, out);
 }
 
 template <int MODE, bool CONTROL>
-void launch_asia(pcf_ctx* ctx, int grid, int B, const AsiaArgs& a) {
-    k_asia<MODE, CONTROL><<<grid, B, 0, ctx->stream>>>(
-        a, ctx->normals.as<double>(), ctx->partials.as<pcf::CSum>(),
-        ctx->counters.as<unsigned int>(), ctx->out.as<double>());
+__global__ __launch_bounds__(1024) void k_asia(const AsiaJob* __restrict__ jobs, int njobs) {
+    __shared__ pcf::CSum lds[16 * (CONTROL ? 4 : 2)];
+    const int j = pcf::find_job(jobs, njobs, blockIdx.x);
+    const AsiaJob& J = jobs[j];
+    asia_body<MODE, CONTROL>(J.a, J.hz, J.partials, J.counter, J.out, blockIdx.x - J.first_block,
+                             J.nblk, lds);
 }
 
-}  // namespace
+struct AsiaPrep {
+    AsiaArgs a;
+    int mode, B, nblk;
+    bool control;
+    double disc;
+};
 
-extern "C" int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
-                           int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts_in,
-                           pcf_result* res) {
-    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    const pcf_opts o = pcf_opts_or_default(opts_in);
-    int st = pcf_check_common(ctx, S0, E, r, sigma, T, N, payoff_fun);
+int asia_prepare(pcf_ctx* ctx, const pcf_request& q, AsiaPrep& p) {
+    const pcf_opts& o = q.opts;
+    int st = pcf_check_common(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun);
     if (st) return st;
     if ((st = pcf_check_opts(ctx, &o))) return st;
+    const int M = q.M;
     if (M <= 0) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "M must be positive (M=%d)", M);
-    PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    *res = pcf_result{};
-
-    const double dt = (double)T / (double)M;
+    const double dt = (double)q.T / (double)M;
     const double sdt = std::sqrt(dt);
     const double log2e = 1.4426950408889634;
-    AsiaArgs a{};
-    a.S0 = S0;
-    a.E = E;
-    a.pf = payoff_fun;
-    a.a1 = 1.0 + r * dt / 2;
-    a.a2 = sigma * sdt / 2;
-    a.mu = (r - 0.5 * sigma * sigma) * dt;
-    a.vol = sigma * sdt;
-    a.h_c1 = 1 + r * dt / 2;
-    a.h_mu = (r - libm_pow(sigma, 2) / 2) * dt;
-    a.sigma = sigma;
-    a.S0f = (float)S0;
-    a.Ef = (float)E;
-    a.pff = (float)payoff_fun;
+    AsiaArgs& a = p.a;
+    a = AsiaArgs{};
+    a.S0 = q.S0;
+    a.E = q.E;
+    a.pf = q.payoff_fun;
+    a.a1 = 1.0 + q.r * dt / 2;
+    a.a2 = q.sigma * sdt / 2;
+    a.mu = (q.r - 0.5 * q.sigma * q.sigma) * dt;
+    a.vol = q.sigma * sdt;
+    a.h_c1 = 1 + q.r * dt / 2;
+    a.h_mu = (q.r - libm_pow(q.sigma, 2) / 2) * dt;
+    a.sigma = q.sigma;
+    a.S0f = (float)q.S0;
+    a.Ef = (float)q.E;
+    a.pff = (float)q.payoff_fun;
     a.a1f = (float)a.a1;
     a.a2f = (float)a.a2;
     a.mu2f = (float)(a.mu * log2e);
     a.vol2f = (float)(a.vol * log2e);
     a.invM = 1.0 / (double)M;
-    a.N = N;
+    a.N = q.N;
     a.M = M;
     a.seed = o.seed;
+    p.mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
+    p.control = o.control != 0 && p.mode != kHost;
+    int B = 256, ipt = 1;
+    pcf_tuned_shape(PCF_KIND_ASIA, p.mode, q.N, &B, &ipt);
+    if (o.block_size) B = o.block_size;
+    if (o.items_per_thread) ipt = o.items_per_thread;
+    p.B = B;
+    p.nblk = pcf_blocks_for(q.N, B, ipt);
+    p.disc = std::exp(-q.r * q.T);
+    return PCF_OK;
+}
 
-    const int mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
-    const bool control = o.control != 0 && mode != kHost;
-    const int B = o.block_size ? o.block_size : 256;
-    const int ipt = o.items_per_thread ? o.items_per_thread : 1;
-    const int grid = (int)std::max<int64_t>(
-        1, std::min<int64_t>((N + (int64_t)B * ipt - 1) / ((int64_t)B * ipt), 1 << 20));
+template <int MODE, bool CONTROL>
+void launch_asia(pcf_ctx* ctx, unsigned int grid, int B, int njobs) {
+    k_asia<MODE, CONTROL><<<grid, B, 0, ctx->stream>>>(ctx->jobs.as<AsiaJob>(), njobs);
+}
+
+}  // namespace
+
+uint64_t pcf_asia_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status) {
+    AsiaPrep p;
+    *status = asia_prepare(ctx, q, p);
+    if (*status) return 0;
+    return pcf_make_batch_key(PCF_KIND_ASIA, p.mode, p.control ? 1 : 0, p.B);
+}
+
+int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status) {
+    if (n < 1 || n > kMaxBatch) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "batch size %d", n);
+    PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    AsiaPrep prep[kMaxBatch];
+    int live[kMaxBatch], nl = 0;
+    for (int i = 0; i < n; ++i) {
+        res[i] = pcf_result{};
+        status[i] = asia_prepare(ctx, *reqs[i], prep[i]);
+        if (status[i] == PCF_OK) live[nl++] = i;
+    }
+    if (nl == 0) return PCF_OK;
+    const int mode = prep[live[0]].mode, B = prep[live[0]].B;
+    const bool control = prep[live[0]].control;
     const int K = control ? 4 : 2;
-
-    if ((st = pcf_timing_begin(ctx))) return st;
-    if (mode == kHost && (st = pcf_upload_normals(ctx, &o, N * (int64_t)M))) return st;
-    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * K * sizeof(pcf::CSum)));
+    int64_t nparts = 0, nb = 0, nnorm = 0;
+    for (int k = 0; k < nl; ++k) {
+        const AsiaPrep& p = prep[live[k]];
+        if (p.mode != mode || p.B != B || p.control != control)
+            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "mixed batch keys");
+        nparts += (int64_t)p.nblk * K;
+        nb += p.nblk;
+        if (p.mode == kHost) nnorm += reqs[live[k]]->N * (int64_t)reqs[live[k]]->M;
+    }
+    if (nb > (1ll << 30)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "batch too large");
+    int st = pcf_timing_begin(ctx);
+    if (st) return st;
+    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)nparts * sizeof(pcf::CSum)));
+    if (nnorm) PCF_HIP_CHECK(ctx, ctx->normals.ensure((size_t)nnorm * sizeof(double)));
+    AsiaJob* jobs = (AsiaJob*)ctx->h_jobs;
+    int64_t poff = 0, noff = 0;
+    unsigned int boff = 0;
+    for (int k = 0; k < nl; ++k) {
+        const int i = live[k];
+        const AsiaPrep& p = prep[i];
+        AsiaJob& J = jobs[k];
+        J.a = p.a;
+        J.hz = nullptr;
+        if (p.mode == kHost) {
+            const pcf_opts& o = reqs[i]->opts;
+            const int64_t cnt = reqs[i]->N * (int64_t)reqs[i]->M;
+            if (!o.host_normals || o.n_host_normals < cnt)
+                return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "need %lld host normals", (long long)cnt);
+            double* dst = ctx->normals.as<double>() + noff;
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(dst, o.host_normals, (size_t)cnt * sizeof(double),
+                                              hipMemcpyHostToDevice, ctx->stream));
+            J.hz = dst;
+            noff += cnt;
+        }
+        J.partials = ctx->partials.as<pcf::CSum>() + poff;
+        J.counter = ctx->counters.as<unsigned int>() + k;
+        J.out = ctx->out.as<double>() + (size_t)kOutPerJob * k;
+        J.first_block = boff;
+        J.nblk = (unsigned int)p.nblk;
+        poff += (int64_t)p.nblk * K;
+        boff += (unsigned int)p.nblk;
+    }
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(AsiaJob) * nl, hipMemcpyHostToDevice,
+                                      ctx->stream));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (mode == kFp32k) {
-        if (control) launch_asia<kFp32k, true>(ctx, grid, B, a);
-        else launch_asia<kFp32k, false>(ctx, grid, B, a);
+        if (control) launch_asia<kFp32k, true>(ctx, boff, B, nl);
+        else launch_asia<kFp32k, false>(ctx, boff, B, nl);
     } else if (mode == kFp64) {
-        if (control) launch_asia<kFp64, true>(ctx, grid, B, a);
-        else launch_asia<kFp64, false>(ctx, grid, B, a);
+        if (control) launch_asia<kFp64, true>(ctx, boff, B, nl);
+        else launch_asia<kFp64, false>(ctx, boff, B, nl);
     } else {
-        launch_asia<kHost, false>(ctx, grid, B, a);
+        launch_asia<kHost, false>(ctx, boff, B, nl);
     }
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
-    if ((st = pcf_timing_end(ctx, control ? 8 : 4, res))) return st;
-
-    const double* h = ctx->h_out;
-    const double disc = std::exp(-r * T);
-    // The reference returns exp(-r*T)*result/(double)N (src/mc_asia.cpp:39).
-    res->price = disc * h[0] / (double)N;
-    res->mean = h[2];
-    res->variance = h[3];
-    res->std_err = disc * std::sqrt(h[3] / (double)N);
-    if (control) {
-        res->control_price = disc * h[4] / (double)N;
-        res->control_se = disc * std::sqrt(h[7] / (double)N);
+    pcf_result timing{};
+    if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
+    for (int k = 0; k < nl; ++k) {
+        const int i = live[k];
+        const AsiaPrep& p = prep[i];
+        const double* h = ctx->h_out + (size_t)kOutPerJob * k;
+        const int64_t N = reqs[i]->N;
+        const int M = reqs[i]->M;
+        pcf_result& r = res[i];
+        // The reference returns exp(-r*T)*result/(double)N (src/mc_asia.cpp:39).
+        r.price = p.disc * h[0] / (double)N;
+        r.mean = h[2];
+        r.variance = h[3];
+        r.std_err = p.disc * std::sqrt(h[3] / (double)N);
+        if (p.control) {
+            r.control_price = p.disc * h[4] / (double)N;
+            r.control_se = p.disc * std::sqrt(h[7] / (double)N);
+        }
+        r.n_paths = N;
+        r.n_steps = M;
+        r.kernel_ms = timing.kernel_ms;
+        r.total_ms = timing.total_ms;
+        // 14 algorithmic flops per path-step: normal 6, I update 2 FMAs (4),
+        // S update FMA + exp + mul (4).  No HBM traffic in the Philox forms.
+        r.flops = 14.0 * (double)N * (double)M;
+        r.bytes = p.mode == kHost ? 8.0 * (double)N * (double)M : 0.0;
+        r.grid_blocks = p.nblk;
+        r.block_size = B;
+        r.batch_size = nl;
     }
-    res->n_paths = N;
-    res->n_steps = M;
-    // 14 algorithmic flops per path-step: normal 6, I update 2 FMAs (4),
-    // S update FMA + exp + mul (4).  No HBM traffic in the Philox forms.
-    res->flops = 14.0 * (double)N * (double)M;
-    res->bytes = mode == kHost ? 8.0 * (double)N * (double)M : 0.0;
-    res->grid_blocks = grid;
-    res->block_size = B;
     return PCF_OK;
+}
+
+extern "C" int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                           int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
+                           pcf_result* res) {
+    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    pcf_request q = pcf_make_request(PCF_KIND_ASIA, S0, E, r, sigma, T, N, payoff_fun, opts);
+    q.M = M;
+    const pcf_request* qp = &q;
+    int status = PCF_OK;
+    int st = pcf_asia_run(ctx, 1, &qp, res, &status);
+    return st ? st : status;
 }

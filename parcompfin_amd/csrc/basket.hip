@@ -17,6 +17,9 @@
 //    16 accumulator registers, halves combined with one lane exchange.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "pcf_internal.hpp"
@@ -58,20 +61,34 @@ struct BasketLayout {
     }
 };
 
+struct BasketJob {
+    BasketArgs a;
+    const void* Lm;   // FMA: d x d factor (Real); MFMA: 32 x 32 zero-padded FP32 factor
+    const void* c0;
+    const void* c1;
+    const void* ws;
+    const double* hz;
+    pcf::CSum* partials;
+    unsigned int* counter;
+    double* out;
+    unsigned int first_block, nblk;
+};
+static_assert(sizeof(BasketJob) <= kJobBytes, "job record exceeds its staging slot");
+
 template <typename Real, int DMAX, int MODE, bool FULL>
-__global__ __launch_bounds__(256) void k_basket_fma(BasketArgs a, const Real* __restrict__ Lm,
-                                                    const Real* __restrict__ c0,
-                                                    const Real* __restrict__ c1,
-                                                    const Real* __restrict__ ws,
-                                                    const double* __restrict__ hz,
-                                                    pcf::CSum* __restrict__ partials,
-                                                    unsigned int* __restrict__ counter,
-                                                    double* __restrict__ out) {
-    __shared__ pcf::CSum lds[4 * 2];
+__device__ __forceinline__ void basket_fma_body(const BasketArgs& a, const Real* __restrict__ Lm,
+                                                const Real* __restrict__ c0,
+                                                const Real* __restrict__ c1,
+                                                const Real* __restrict__ ws,
+                                                const double* __restrict__ hz,
+                                                pcf::CSum* __restrict__ partials,
+                                                unsigned int* __restrict__ counter,
+                                                double* __restrict__ out, unsigned int blk,
+                                                unsigned int nblk, pcf::CSum* lds) {
     const int d = a.d;
     double s = 0.0, s2 = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.N; p += stride) {
+    const int64_t stride = (int64_t)nblk * blockDim.x;
+    for (int64_t p = (int64_t)blk * blockDim.x + threadIdx.x; p < a.N; p += stride) {
         Real z[DMAX];
         if (MODE == kHost) {
 #pragma unroll
@@ -124,25 +141,30 @@ __global__ __launch_bounds__(256) void k_basket_fma(BasketArgs a, const Real* __
     pcf::Acc<2> acc, tot;
     acc.v[0] = {s, 0.0};
     acc.v[1] = {s2, 0.0};
-    if (pcf::grid_reduce<2>(acc, partials, counter, lds, tot) && threadIdx.x == 0) {
-        const double sum = tot.v[0].s + tot.v[0].c, sq = tot.v[1].s + tot.v[1].c, n = (double)a.N;
-        out[0] = sum;
-        out[1] = sq;
-        out[2] = sum / n;
-        out[3] = a.N > 1 ? (sq - sum * (sum / n)) / (n - 1.0) : 0.0;
-    }
+    if (pcf::grid_reduce<2>(acc, partials, counter, lds, tot, blk, nblk) && threadIdx.x == 0)
+        pcf::write_moments<1>(tot, a.N, out);
+}
+
+template <typename Real, int DMAX, int MODE, bool FULL>
+__global__ __launch_bounds__(256) void k_basket_fma(const BasketJob* __restrict__ jobs, int njobs) {
+    __shared__ pcf::CSum lds[4 * 2];
+    const int j = pcf::find_job(jobs, njobs, blockIdx.x);
+    const BasketJob& J = jobs[j];
+    basket_fma_body<Real, DMAX, MODE, FULL>(J.a, (const Real*)J.Lm, (const Real*)J.c0, (const Real*)J.c1,
+                                            (const Real*)J.ws, J.hz, J.partials, J.counter, J.out,
+                                            blockIdx.x - J.first_block, J.nblk, lds);
 }
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-__global__ __launch_bounds__(256) void k_basket_mfma(BasketArgs a, const float* __restrict__ Lpad,
-                                                     const float* __restrict__ c0,
-                                                     const float* __restrict__ c1,
-                                                     const float* __restrict__ ws,
-                                                     pcf::CSum* __restrict__ partials,
-                                                     unsigned int* __restrict__ counter,
-                                                     double* __restrict__ out) {
-    __shared__ pcf::CSum lds[4 * 2];
+__device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const float* __restrict__ Lpad,
+                                                 const float* __restrict__ c0,
+                                                 const float* __restrict__ c1,
+                                                 const float* __restrict__ ws,
+                                                 pcf::CSum* __restrict__ partials,
+                                                 unsigned int* __restrict__ counter,
+                                                 double* __restrict__ out, unsigned int blk,
+                                                 unsigned int nblk, pcf::CSum* lds) {
     const int lane = threadIdx.x & 63;
     const int j = lane & 31;  // path within the tile; also the A-operand row (asset)
     const int h = lane >> 5;  // k half fed by this lane
@@ -160,11 +182,11 @@ __global__ __launch_bounds__(256) void k_basket_mfma(BasketArgs a, const float* 
         e1[r] = c1[i];
         ew[r] = ws[i];
     }
-    const int nblk = (a.d + 7) >> 3;  // k blocks of 8 actually used
+    const int kblk = (a.d + 7) >> 3;  // k blocks of 8 actually used
     double s = 0.0, s2 = 0.0;
     const int64_t ntiles = (a.N + 31) >> 5;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blk * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)nblk * blockDim.x) >> 6;
     for (int64_t t = wave; t < ntiles; t += nwaves) {
         const int64_t p = (t << 5) + j;
         v16f acc;
@@ -172,7 +194,7 @@ __global__ __launch_bounds__(256) void k_basket_mfma(BasketArgs a, const float* 
         for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
-            if (cc < nblk) {
+            if (cc < kblk) {
                 float z[4];
                 pcf::normals4<float>(a.seed, PCF_STREAM_BASKET, (uint64_t)p, (uint32_t)(2 * cc + h), z);
 #pragma unroll
@@ -194,13 +216,16 @@ __global__ __launch_bounds__(256) void k_basket_mfma(BasketArgs a, const float* 
     pcf::Acc<2> acc2, tot;
     acc2.v[0] = {s, 0.0};
     acc2.v[1] = {s2, 0.0};
-    if (pcf::grid_reduce<2>(acc2, partials, counter, lds, tot) && threadIdx.x == 0) {
-        const double sum = tot.v[0].s + tot.v[0].c, sq = tot.v[1].s + tot.v[1].c, n = (double)a.N;
-        out[0] = sum;
-        out[1] = sq;
-        out[2] = sum / n;
-        out[3] = a.N > 1 ? (sq - sum * (sum / n)) / (n - 1.0) : 0.0;
-    }
+    if (pcf::grid_reduce<2>(acc2, partials, counter, lds, tot, blk, nblk) && threadIdx.x == 0)
+        pcf::write_moments<1>(tot, a.N, out);
+}
+
+__global__ __launch_bounds__(256) void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
+    __shared__ pcf::CSum lds[4 * 2];
+    const int j = pcf::find_job(jobs, njobs, blockIdx.x);
+    const BasketJob& J = jobs[j];
+    basket_mfma_body(J.a, (const float*)J.Lm, (const float*)J.c0, (const float*)J.c1, (const float*)J.ws,
+                     J.partials, J.counter, J.out, blockIdx.x - J.first_block, J.nblk, lds);
 }
 
 // ---- host linear algebra (FP64) ---------------------------------------------
@@ -267,176 +292,348 @@ bool eigen_factor(int d, const std::vector<double>& C, std::vector<double>& F) {
     return true;
 }
 
+// ---- host: factor cache, preparation, batched launch ------------------------
+
+// Process-wide cache of factorised baskets (Cholesky or eigen factor plus the
+// per-asset constants), keyed by a hash of everything they depend on.
+struct BasketConsts {
+    std::vector<char> blob;
+    BasketLayout lay;
+    bool full;
+};
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::mutex& consts_mu() {
+    static std::mutex m;
+    return m;
+}
+std::map<uint64_t, std::shared_ptr<BasketConsts>>& consts_cache() {
+    static std::map<uint64_t, std::shared_ptr<BasketConsts>> c;
+    return c;
+}
+
+struct BasketPrep {
+    BasketArgs a;
+    int mode, B, nblk, dmax;
+    bool mfma, full;
+    uint64_t key;
+    std::shared_ptr<BasketConsts> consts;
+    double disc;
+};
+
+int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
+    const pcf_opts& o = q.opts;
+    int st = pcf_check_common(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun);
+    if (st) return st;
+    if ((st = pcf_check_opts(ctx, &o))) return st;
+    const int d = q.assets;
+    if (d < 1 || d > kMaxAssets)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "assets must be in [1, %d] (got %d)", kMaxAssets, d);
+    const pcf_basket_ext* ext = q.basket;
+    // Key of the constants: d, rho, r, T, sqrt(T) convention, scalar sigma/S0
+    // and every extension array.
+    uint64_t h = 1469598103934665603ull;
+    const int refx = o.ref_exact ? 1 : 0;
+    h = fnv(h, &d, sizeof d);
+    h = fnv(h, &q.rho, sizeof q.rho);
+    h = fnv(h, &q.r, sizeof q.r);
+    h = fnv(h, &q.T, sizeof q.T);
+    h = fnv(h, &q.sigma, sizeof q.sigma);
+    h = fnv(h, &q.S0, sizeof q.S0);
+    h = fnv(h, &refx, sizeof refx);
+    const double* arrs[4] = {ext ? ext->sigma : nullptr, ext ? ext->S0 : nullptr,
+                             ext ? ext->weights : nullptr, ext ? ext->corr : nullptr};
+    for (int k = 0; k < 4; ++k) {
+        const int present = arrs[k] ? 1 : 0;
+        h = fnv(h, &present, sizeof present);
+        if (arrs[k]) h = fnv(h, arrs[k], sizeof(double) * (k == 3 ? (size_t)d * d : (size_t)d));
+    }
+    std::shared_ptr<BasketConsts> c;
+    {
+        std::lock_guard<std::mutex> lk(consts_mu());
+        auto it = consts_cache().find(h);
+        if (it != consts_cache().end()) c = it->second;
+    }
+    if (!c) {
+        // Correlation matrix: ext->corr or equicorrelation rho (include/mvn.h:55-60).
+        std::vector<double> C((size_t)d * d);
+        for (int i = 0; i < d; ++i)
+            for (int k = 0; k < d; ++k) {
+                const double v = (ext && ext->corr) ? ext->corr[i * d + k] : (i != k ? q.rho : 1.0);
+                if (!std::isfinite(v)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "non-finite correlation");
+                C[(size_t)i * d + k] = v;
+            }
+        for (int i = 0; i < d; ++i)
+            for (int k = 0; k < i; ++k)
+                if (std::fabs(C[i * d + k] - C[k * d + i]) > 1e-12)
+                    return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "correlation matrix is not symmetric");
+        std::vector<double> L;
+        bool full = false;
+        if (!cholesky(d, C, L)) {
+            if (!eigen_factor(d, C, L))
+                return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "correlation matrix is not positive semi-definite");
+            full = true;
+        }
+        c = std::make_shared<BasketConsts>();
+        c->full = full;
+        c->lay = BasketLayout::make(d);
+        c->blob.assign(c->lay.bytes, 0);
+        char* hb = c->blob.data();
+        const BasketLayout& lay = c->lay;
+        double* L64 = (double*)(hb + lay.off_L64);
+        double* c064 = (double*)(hb + lay.off_c064);
+        double* c164 = (double*)(hb + lay.off_c164);
+        double* ws64 = (double*)(hb + lay.off_ws64);
+        float* Lf = (float*)(hb + lay.off_Lf);
+        float* Lpad = (float*)(hb + lay.off_Lpad);
+        float* c0f = (float*)(hb + lay.off_c0f);
+        float* c1f = (float*)(hb + lay.off_c1f);
+        float* wsf = (float*)(hb + lay.off_wsf);
+        const double log2e = 1.4426950408889634;
+        const double sqT = refx ? 1.0 : std::sqrt(q.T);
+        const double w_i = 1.0 / (double)d;
+        for (int i = 0; i < d * d; ++i) {
+            L64[i] = L[i];
+            Lf[i] = (float)L[i];
+        }
+        if (d <= 32)
+            for (int i = 0; i < d; ++i)
+                for (int k = 0; k < d; ++k) Lpad[i * 32 + k] = (float)L[i * d + k];
+        for (int i = 0; i < d; ++i) {
+            const double sg = (ext && ext->sigma) ? ext->sigma[i] : q.sigma;
+            const double s0 = (ext && ext->S0) ? ext->S0[i] : q.S0;
+            const double wt = (ext && ext->weights) ? ext->weights[i] : w_i;
+            if (!(sg >= 0) || !std::isfinite(sg) || !std::isfinite(s0) || !std::isfinite(wt))
+                return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "bad per-asset parameter at %d", i);
+            // (w_i*S0)*exp((r-pow(sigma,2)/2)*T + sigma*B)  (src/mc_eur_multi.cpp:30)
+            c064[i] = (q.r - libm_pow(sg, 2) / 2) * q.T;
+            c164[i] = sg * sqT;
+            ws64[i] = wt * s0;
+            c0f[i] = (float)(c064[i] * log2e);
+            c1f[i] = (float)(c164[i] * log2e);
+            wsf[i] = (float)ws64[i];
+        }
+        std::lock_guard<std::mutex> lk(consts_mu());
+        if (consts_cache().size() > 1024) consts_cache().clear();
+        consts_cache()[h] = c;
+    }
+    p.consts = c;
+    p.key = h;
+    p.full = c->full;
+    p.mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
+    if (o.kernel_variant == 2 && !(p.mode == kFp32k && d >= 16 && d <= 32))
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "MFMA basket kernel needs FP32/Philox and 16 <= assets <= 32");
+    p.mfma = p.mode == kFp32k && d >= 16 && d <= 32 && o.kernel_variant != 1;
+    p.dmax = d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : 64;
+    BasketArgs& a = p.a;
+    a = BasketArgs{};
+    a.E = q.E;
+    a.pf = q.payoff_fun;
+    a.Ef = (float)q.E;
+    a.pff = (float)q.payoff_fun;
+    a.N = q.N;
+    a.d = d;
+    a.seed = o.seed;
+    int B = 256, ipt = p.mfma ? 2 : 1;
+    pcf_tuned_shape(PCF_KIND_BASKET, p.mode * 2 + (p.mfma ? 1 : 0), q.N, &B, &ipt);
+    if (o.block_size) B = o.block_size;
+    if (o.items_per_thread) ipt = o.items_per_thread;
+    p.B = B;
+    if (p.mfma) {
+        // ipt = 32-path tiles per wavefront
+        const int64_t ntiles = (q.N + 31) / 32;
+        p.nblk = pcf_blocks_for(ntiles, B / 64, ipt);
+    } else {
+        p.nblk = pcf_blocks_for(q.N, B, ipt);
+    }
+    p.disc = std::exp(-q.r * q.T);
+    return PCF_OK;
+}
+
+// Device-resident constants: fixed-size slots in ctx->basket_L, LRU by stamp.
+constexpr size_t kConstSlot = 64 * 1024;
+constexpr int kConstSlots = kMaxBatch;
+
+int resident_consts(pcf_ctx* ctx, const BasketPrep& p, uint64_t batch_stamp, char** dev) {
+    auto& bc = ctx->basket_cache;
+    if (bc.key.empty()) {
+        bc.key.assign(kConstSlots, 0);
+        bc.stamp.assign(kConstSlots, 0);
+        PCF_HIP_CHECK(ctx, ctx->basket_L.ensure(kConstSlot * kConstSlots));
+    }
+    auto it = bc.where.find(p.key);
+    int slot;
+    if (it != bc.where.end()) {
+        slot = it->second;
+    } else {
+        slot = -1;
+        for (int s = 0; s < kConstSlots; ++s)
+            if (bc.stamp[s] < batch_stamp && (slot < 0 || bc.stamp[s] < bc.stamp[slot])) slot = s;
+        if (slot < 0) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "too many distinct baskets in one batch");
+        if (bc.key[slot]) bc.where.erase(bc.key[slot]);
+        bc.key[slot] = p.key;
+        bc.where[p.key] = slot;
+        if (p.consts->blob.size() > kConstSlot) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "basket too large");
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->basket_L.as<char>() + kConstSlot * slot, p.consts->blob.data(),
+                                          p.consts->blob.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    bc.stamp[slot] = ++bc.clock;
+    *dev = ctx->basket_L.as<char>() + kConstSlot * slot;
+    return PCF_OK;
+}
+
 template <typename Real, int DMAX, int MODE>
-void launch_fma(pcf_ctx* ctx, int grid, int B, const BasketArgs& a, bool full,
-                const BasketLayout& lay, int d) {
-    char* base = ctx->basket_L.as<char>();
-    const bool f64 = sizeof(Real) == 8;
-    const Real* Lm = (const Real*)(base + (f64 ? lay.off_L64 : lay.off_Lf));
-    const Real* c0 = (const Real*)(base + (f64 ? lay.off_c064 : lay.off_c0f));
-    const Real* c1 = (const Real*)(base + (f64 ? lay.off_c164 : lay.off_c1f));
-    const Real* ws = (const Real*)(base + (f64 ? lay.off_ws64 : lay.off_wsf));
-    (void)d;
-    auto* part = ctx->partials.as<pcf::CSum>();
-    auto* cnt = ctx->counters.as<unsigned int>();
-    auto* out = ctx->out.as<double>();
-    const double* hz = ctx->normals.as<double>();
-    if (full)
-        k_basket_fma<Real, DMAX, MODE, true><<<grid, B, 0, ctx->stream>>>(a, Lm, c0, c1, ws, hz, part, cnt, out);
-    else
-        k_basket_fma<Real, DMAX, MODE, false><<<grid, B, 0, ctx->stream>>>(a, Lm, c0, c1, ws, hz, part, cnt, out);
+void launch_fma(pcf_ctx* ctx, unsigned int grid, int B, int nj, bool full) {
+    const BasketJob* dj = ctx->jobs.as<BasketJob>();
+    if (full) k_basket_fma<Real, DMAX, MODE, true><<<grid, B, 0, ctx->stream>>>(dj, nj);
+    else k_basket_fma<Real, DMAX, MODE, false><<<grid, B, 0, ctx->stream>>>(dj, nj);
 }
 
 template <typename Real, int MODE>
-void launch_fma_d(pcf_ctx* ctx, int grid, int B, const BasketArgs& a, bool full,
-                  const BasketLayout& lay, int d) {
-    if (d <= 4) launch_fma<Real, 4, MODE>(ctx, grid, B, a, full, lay, d);
-    else if (d <= 8) launch_fma<Real, 8, MODE>(ctx, grid, B, a, full, lay, d);
-    else if (d <= 16) launch_fma<Real, 16, MODE>(ctx, grid, B, a, full, lay, d);
-    else if (d <= 32) launch_fma<Real, 32, MODE>(ctx, grid, B, a, full, lay, d);
-    else launch_fma<Real, 64, MODE>(ctx, grid, B, a, full, lay, d);
+void launch_fma_d(pcf_ctx* ctx, unsigned int grid, int B, int nj, bool full, int dmax) {
+    if (dmax == 4) launch_fma<Real, 4, MODE>(ctx, grid, B, nj, full);
+    else if (dmax == 8) launch_fma<Real, 8, MODE>(ctx, grid, B, nj, full);
+    else if (dmax == 16) launch_fma<Real, 16, MODE>(ctx, grid, B, nj, full);
+    else if (dmax == 32) launch_fma<Real, 32, MODE>(ctx, grid, B, nj, full);
+    else launch_fma<Real, 64, MODE>(ctx, grid, B, nj, full);
 }
 
 }  // namespace
 
-extern "C" int pcf_mc_eur_multi(pcf_ctx* ctx, double S0, double E, double r, double sigma,
-                                double T, int64_t N, double payoff_fun, int32_t assets, double rho,
-                                const pcf_basket_ext* ext, const pcf_opts* opts_in,
-                                pcf_result* res) {
-    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    const pcf_opts o = pcf_opts_or_default(opts_in);
-    int st = pcf_check_common(ctx, S0, E, r, sigma, T, N, payoff_fun);
-    if (st) return st;
-    if ((st = pcf_check_opts(ctx, &o))) return st;
-    const int d = assets;
-    if (d < 1 || d > kMaxAssets)
-        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "assets must be in [1, %d] (got %d)", kMaxAssets, d);
+uint64_t pcf_basket_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status) {
+    BasketPrep p;
+    *status = basket_prepare(ctx, q, p);
+    if (*status) return 0;
+    const int variant = p.mfma ? 1 : (2 + (p.full ? 1 : 0) + 2 * p.dmax);
+    return pcf_make_batch_key(PCF_KIND_BASKET, p.mode, variant, p.B);
+}
+
+int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status) {
+    if (n < 1 || n > kMaxBatch) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "batch size %d", n);
     PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    *res = pcf_result{};
-
-    // Correlation matrix: ext->corr or equicorrelation rho (include/mvn.h:55-60).
-    std::vector<double> C((size_t)d * d);
-    for (int i = 0; i < d; ++i)
-        for (int k = 0; k < d; ++k) {
-            double v = (ext && ext->corr) ? ext->corr[i * d + k] : (i != k ? rho : 1.0);
-            if (!std::isfinite(v)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "non-finite correlation");
-            C[(size_t)i * d + k] = v;
+    static thread_local BasketPrep prep[kMaxBatch];
+    int live[kMaxBatch], nl = 0;
+    for (int i = 0; i < n; ++i) {
+        res[i] = pcf_result{};
+        status[i] = basket_prepare(ctx, *reqs[i], prep[i]);
+        if (status[i] == PCF_OK) live[nl++] = i;
+    }
+    if (nl == 0) return PCF_OK;
+    const BasketPrep& p0 = prep[live[0]];
+    int64_t nparts = 0, nb = 0, nnorm = 0;
+    for (int k = 0; k < nl; ++k) {
+        const BasketPrep& p = prep[live[k]];
+        if (p.mode != p0.mode || p.B != p0.B || p.mfma != p0.mfma || p.dmax != p0.dmax || p.full != p0.full)
+            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "mixed batch keys");
+        nparts += (int64_t)p.nblk * 2;
+        nb += p.nblk;
+        if (p.mode == kHost) nnorm += reqs[live[k]]->N * (int64_t)reqs[live[k]]->assets;
+    }
+    if (nb > (1ll << 30)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "batch too large");
+    int st = pcf_timing_begin(ctx);
+    if (st) return st;
+    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)nparts * sizeof(pcf::CSum)));
+    if (nnorm) PCF_HIP_CHECK(ctx, ctx->normals.ensure((size_t)nnorm * sizeof(double)));
+    BasketJob* jobs = (BasketJob*)ctx->h_jobs;
+    const uint64_t batch_stamp = ctx->basket_cache.clock + 1;
+    int64_t poff = 0, noff = 0;
+    unsigned int boff = 0;
+    for (int k = 0; k < nl; ++k) {
+        const int i = live[k];
+        const BasketPrep& p = prep[i];
+        char* dev = nullptr;
+        if ((st = resident_consts(ctx, p, batch_stamp, &dev))) return st;
+        const BasketLayout& lay = p.consts->lay;
+        const bool f64 = p.mode != kFp32k;
+        BasketJob& J = jobs[k];
+        J.a = p.a;
+        J.Lm = dev + (p.mfma ? lay.off_Lpad : (f64 ? lay.off_L64 : lay.off_Lf));
+        J.c0 = dev + (f64 ? lay.off_c064 : lay.off_c0f);
+        J.c1 = dev + (f64 ? lay.off_c164 : lay.off_c1f);
+        J.ws = dev + (f64 ? lay.off_ws64 : lay.off_wsf);
+        J.hz = nullptr;
+        if (p.mode == kHost) {
+            const pcf_opts& o = reqs[i]->opts;
+            const int64_t cnt = reqs[i]->N * (int64_t)reqs[i]->assets;
+            if (!o.host_normals || o.n_host_normals < cnt)
+                return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "need %lld host normals", (long long)cnt);
+            double* dst = ctx->normals.as<double>() + noff;
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(dst, o.host_normals, (size_t)cnt * sizeof(double),
+                                              hipMemcpyHostToDevice, ctx->stream));
+            J.hz = dst;
+            noff += cnt;
         }
-    for (int i = 0; i < d; ++i)
-        for (int k = 0; k < i; ++k)
-            if (std::fabs(C[i * d + k] - C[k * d + i]) > 1e-12)
-                return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "correlation matrix is not symmetric");
-    std::vector<double> L;
-    bool full = false;
-    if (!cholesky(d, C, L)) {
-        if (!eigen_factor(d, C, L))
-            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "correlation matrix is not positive semi-definite");
-        full = true;
+        J.partials = ctx->partials.as<pcf::CSum>() + poff;
+        J.counter = ctx->counters.as<unsigned int>() + k;
+        J.out = ctx->out.as<double>() + (size_t)kOutPerJob * k;
+        J.first_block = boff;
+        J.nblk = (unsigned int)p.nblk;
+        poff += (int64_t)p.nblk * 2;
+        boff += (unsigned int)p.nblk;
     }
-    // Per-asset constants.
-    const BasketLayout lay = BasketLayout::make(d);
-    std::vector<char> hb(lay.bytes, 0);
-    double* L64 = (double*)(hb.data() + lay.off_L64);
-    double* c064 = (double*)(hb.data() + lay.off_c064);
-    double* c164 = (double*)(hb.data() + lay.off_c164);
-    double* ws64 = (double*)(hb.data() + lay.off_ws64);
-    float* Lf = (float*)(hb.data() + lay.off_Lf);
-    float* Lpad = (float*)(hb.data() + lay.off_Lpad);
-    float* c0f = (float*)(hb.data() + lay.off_c0f);
-    float* c1f = (float*)(hb.data() + lay.off_c1f);
-    float* wsf = (float*)(hb.data() + lay.off_wsf);
-    const double log2e = 1.4426950408889634;
-    const double sqT = o.ref_exact ? 1.0 : std::sqrt(T);
-    const double w_i = 1.0 / (double)d;
-    for (int i = 0; i < d * d; ++i) {
-        L64[i] = L[i];
-        Lf[i] = (float)L[i];
-    }
-    if (d <= 32)
-        for (int i = 0; i < d; ++i)
-            for (int k = 0; k < d; ++k) Lpad[i * 32 + k] = (float)L[i * d + k];
-    for (int i = 0; i < d; ++i) {
-        const double sg = (ext && ext->sigma) ? ext->sigma[i] : sigma;
-        const double s0 = (ext && ext->S0) ? ext->S0[i] : S0;
-        const double wt = (ext && ext->weights) ? ext->weights[i] : w_i;
-        if (!(sg >= 0) || !std::isfinite(s0) || !std::isfinite(wt))
-            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "bad per-asset parameter at %d", i);
-        c064[i] = (r - libm_pow(sg, 2) / 2) * T;
-        c164[i] = sg * sqT;
-        ws64[i] = wt * s0;
-        c0f[i] = (float)(c064[i] * log2e);
-        c1f[i] = (float)(c164[i] * log2e);
-        wsf[i] = (float)ws64[i];
-    }
-
-    const int mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
-    if (o.kernel_variant == 2 && !(mode == kFp32k && d >= 16 && d <= 32))
-        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "MFMA basket kernel needs FP32/Philox and 16 <= assets <= 32");
-    const bool use_mfma = mode == kFp32k && d >= 16 && d <= 32 && o.kernel_variant != 1;
-    BasketArgs a{};
-    a.E = E;
-    a.pf = payoff_fun;
-    a.Ef = (float)E;
-    a.pff = (float)payoff_fun;
-    a.N = N;
-    a.d = d;
-    a.seed = o.seed;
-
-    const int B = o.block_size ? o.block_size : 256;
-    int grid;
-    if (use_mfma) {
-        const int64_t ntiles = (N + 31) / 32;
-        const int waves_per_block = B / 64;
-        const int ipt = o.items_per_thread ? o.items_per_thread : 2;  // tiles per wave
-        grid = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + (int64_t)waves_per_block * ipt - 1) /
-                                                               ((int64_t)waves_per_block * ipt),
-                                                           1 << 20));
-    } else {
-        const int ipt = o.items_per_thread ? o.items_per_thread : 1;
-        grid = (int)std::max<int64_t>(
-            1, std::min<int64_t>((N + (int64_t)B * ipt - 1) / ((int64_t)B * ipt), 1 << 20));
-    }
-
-    if ((st = pcf_timing_begin(ctx))) return st;
-    PCF_HIP_CHECK(ctx, ctx->basket_L.ensure(lay.bytes));
-    PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->basket_L.ptr, hb.data(), lay.bytes, hipMemcpyHostToDevice,
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(BasketJob) * nl, hipMemcpyHostToDevice,
                                       ctx->stream));
-    if (mode == kHost && (st = pcf_upload_normals(ctx, &o, N * (int64_t)d))) return st;
-    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * 2 * sizeof(pcf::CSum)));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
-    if (use_mfma) {
-        char* base = ctx->basket_L.as<char>();
-        k_basket_mfma<<<grid, B, 0, ctx->stream>>>(
-            a, (const float*)(base + lay.off_Lpad), (const float*)(base + lay.off_c0f),
-            (const float*)(base + lay.off_c1f), (const float*)(base + lay.off_wsf),
-            ctx->partials.as<pcf::CSum>(), ctx->counters.as<unsigned int>(), ctx->out.as<double>());
-    } else if (mode == kFp32k) {
-        launch_fma_d<float, kFp32k>(ctx, grid, B, a, full, lay, d);
-    } else if (mode == kFp64) {
-        launch_fma_d<double, kFp64>(ctx, grid, B, a, full, lay, d);
-    } else {
-        launch_fma_d<double, kHost>(ctx, grid, B, a, full, lay, d);
-    }
+    if (p0.mfma)
+        k_basket_mfma<<<boff, p0.B, 0, ctx->stream>>>(ctx->jobs.as<BasketJob>(), nl);
+    else if (p0.mode == kFp32k)
+        launch_fma_d<float, kFp32k>(ctx, boff, p0.B, nl, p0.full, p0.dmax);
+    else if (p0.mode == kFp64)
+        launch_fma_d<double, kFp64>(ctx, boff, p0.B, nl, p0.full, p0.dmax);
+    else
+        launch_fma_d<double, kHost>(ctx, boff, p0.B, nl, p0.full, p0.dmax);
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
-    if ((st = pcf_timing_end(ctx, 4, res))) return st;
-
-    const double* h = ctx->h_out;
-    const double disc = std::exp(-r * T);
-    // The reference returns (exp(-r*T)*result)/(double)N (src/mc_eur_multi.cpp:34).
-    res->price = (disc * h[0]) / (double)N;
-    res->mean = h[2];
-    res->variance = h[3];
-    res->std_err = disc * std::sqrt(h[3] / (double)N);
-    res->n_paths = N;
-    res->n_steps = 1;
-    // d^2 + 12 d + 3 algorithmic flops per path: normals 6d, triangular Z L^T
-    // d(d+1), per asset FMA + exp + FMA 5d, payoff 3.
-    res->flops = ((double)d * d + 12.0 * d + 3.0) * (double)N;
-    res->bytes = mode == kHost ? 8.0 * d * (double)N : 0.0;
-    res->grid_blocks = grid;
-    res->block_size = B;
-    res->lsm_fallback = full ? 1 : 0;
+    pcf_result timing{};
+    if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
+    for (int k = 0; k < nl; ++k) {
+        const int i = live[k];
+        const BasketPrep& p = prep[i];
+        const double* h = ctx->h_out + (size_t)kOutPerJob * k;
+        const int64_t N = reqs[i]->N;
+        const double d = reqs[i]->assets;
+        pcf_result& r = res[i];
+        // The reference returns (exp(-r*T)*result)/(double)N (src/mc_eur_multi.cpp:34).
+        r.price = (p.disc * h[0]) / (double)N;
+        r.mean = h[2];
+        r.variance = h[3];
+        r.std_err = p.disc * std::sqrt(h[3] / (double)N);
+        r.n_paths = N;
+        r.n_steps = 1;
+        r.kernel_ms = timing.kernel_ms;
+        r.total_ms = timing.total_ms;
+        // d^2 + 12 d + 3 algorithmic flops per path: normals 6d, triangular
+        // Z L^T d(d+1), per asset FMA + exp + FMA 5d, payoff 3.
+        r.flops = (d * d + 12.0 * d + 3.0) * (double)N;
+        r.bytes = p.mode == kHost ? 8.0 * d * (double)N : 0.0;
+        r.grid_blocks = p.nblk;
+        r.block_size = p.B;
+        r.batch_size = nl;
+        r.lsm_fallback = p.full ? 1 : 0;
+    }
+    for (int k = 0; k < nl; ++k) prep[live[k]].consts.reset();
     return PCF_OK;
+}
+
+extern "C" int pcf_mc_eur_multi(pcf_ctx* ctx, double S0, double E, double r, double sigma,
+                                double T, int64_t N, double payoff_fun, int32_t assets, double rho,
+                                const pcf_basket_ext* ext, const pcf_opts* opts, pcf_result* res) {
+    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    pcf_request q = pcf_make_request(PCF_KIND_BASKET, S0, E, r, sigma, T, N, payoff_fun, opts);
+    q.assets = assets;
+    q.rho = rho;
+    q.basket = ext;
+    const pcf_request* qp = &q;
+    int status = PCF_OK;
+    int st = pcf_basket_run(ctx, 1, &qp, res, &status);
+    return st ? st : status;
 }

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     pcf::Acc<1> ac, tot;
     ac.v[0] = {s, 0.0};
     pcf::csum_add(ac.v[0], s_b);
-    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot) && threadIdx.x == 0) {
+    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         out[0] = tot.v[0].s + tot.v[0].c;
     }
 }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_binom_lgamma(BinomArgs a, pcf::CSum* __
     pcf::Acc<1> ac, tot;
     ac.v[0] = {s, 0.0};
     pcf::csum_add(ac.v[0], s_b);
-    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot) && threadIdx.x == 0)
+    if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0)
         out[0] = tot.v[0].s + tot.v[0].c;
 }
 
@@ -211,5 +211,6 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     res->bytes = chain ? 8.0 * adds : 0.0;
     res->grid_blocks = grid;
     res->block_size = B;
+    res->batch_size = 1;
     return PCF_OK;
 }

@@ -70,6 +70,9 @@ int pcf_ctx_create(int device, pcf_ctx** out) {
         return bail(e, "alloc out");
     if ((e = hipHostMalloc((void**)&ctx->h_out, kOutDoubles * sizeof(double), 0)) != hipSuccess)
         return bail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc(&ctx->h_jobs, kMaxBatch * kJobBytes, 0)) != hipSuccess)
+        return bail(e, "hipHostMalloc jobs");
+    if ((e = ctx->jobs.ensure(kMaxBatch * kJobBytes)) != hipSuccess) return bail(e, "alloc jobs");
     *out = ctx;
     return PCF_OK;
 }
@@ -81,9 +84,10 @@ int pcf_ctx_destroy(pcf_ctx* ctx) {
     pcf_lsm_release(ctx);
     for (DevBuf* b : {&ctx->partials, &ctx->counters, &ctx->out, &ctx->normals, &ctx->logtab,
                       &ctx->prefix, &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef,
-                      &ctx->basket_L, &ctx->scratch})
+                      &ctx->basket_L, &ctx->scratch, &ctx->jobs})
         b->release();
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+    if (ctx->h_jobs) (void)hipHostFree(ctx->h_jobs);
     for (hipEvent_t ev : {ctx->ev_begin, ctx->ev_kbeg, ctx->ev_kend, ctx->ev_end})
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -230,5 +234,6 @@ extern "C" int pcf_struct_size(const char* name) {
     if (n == "pcf_request") return (int)sizeof(pcf_request);
     if (n == "pcf_server_config") return (int)sizeof(pcf_server_config);
     if (n == "pcf_request_timing") return (int)sizeof(pcf_request_timing);
+    if (n == "pcf_tune_entry") return (int)sizeof(pcf_tune_entry);
     return -1;
 }

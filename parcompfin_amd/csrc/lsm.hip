@@ -11,7 +11,7 @@
 //                                 booked cash flow (differs from cf only under
 //                                 the reference's payoff(S-E,E) exercise test)
 //   coef[m] = {flag, c0, c1, c2}  regression result per date
-// Launch sequence (one stream, captured once per shape into a hipGraph):
+// Launch sequence (one stream; built once per shape as a hipGraph and replayed):
 //   k_lsm_paths        store + state init + European control + sums of date M-1
 //   k_lsm_step(m)      exercise decision at m, then sums of date m-1   (m = M-1..2)
 //   k_lsm_final        decision at date 1, then sum of discounted cash flows
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
     a.v[1] = {e1, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         b.out[0] = tot.v[0].s + tot.v[0].c;
         b.out[1] = tot.v[1].s + tot.v[1].c;
         if (M >= 2) {
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
     a.v[1] = {e1, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         b.out[0] = tot.v[0].s + tot.v[0].c;
         b.out[1] = tot.v[1].s + tot.v[1].c;
         if (M >= 2) {
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_lsm_step(LsmBufs b, int m) {
     pcf::Acc<8> a, tot;
 #pragma unroll
     for (int k = 0; k < 8; ++k) a.v[k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<8>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0)
+    if (pcf::grid_reduce<8>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0)
         lsm_solve<REF>(tot, b.coef, m - 1);
 }
 
@@ -449,21 +449,44 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
     pcf::Acc<2> a, tot;
     a.v[0] = {s, 0.0};
     a.v[1] = {s2, 0.0};
-    if (pcf::grid_reduce<2>(a, b.partials, b.counter, lds, tot) && threadIdx.x == 0) {
+    if (pcf::grid_reduce<2>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         b.out[2] = tot.v[0].s + tot.v[0].c;
         b.out[3] = tot.v[1].s + tot.v[1].c;
     }
 }
 
 // ---- host: buffers, graph cache --------------------------------------------
+// Graph shapes depend on the size class of N (capacity 2^ceil(log2 N)), not
+// on N itself: the kernels read N from the parameter block, so any N of a
+// class replays the same captured graph with the same launch shape.
 struct GraphKey {
-    int64_t N;
+    int64_t cap;
     int M, real, src, ref, grid_paths, grid_step, B;
     bool operator<(const GraphKey& o) const {
-        return std::tie(N, M, real, src, ref, grid_paths, grid_step, B) <
-               std::tie(o.N, o.M, o.real, o.src, o.ref, o.grid_paths, o.grid_step, o.B);
+        return std::tie(cap, M, real, src, ref, grid_paths, grid_step, B) <
+               std::tie(o.cap, o.M, o.real, o.src, o.ref, o.grid_paths, o.grid_step, o.B);
     }
 };
+
+struct LsmShape {
+    int64_t cap;
+    int B, ipt, grid_paths, grid_step;
+};
+
+LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o) {
+    LsmShape sh;
+    sh.cap = (int64_t)1 << pcf_size_class(N);
+    if (sh.cap < N) sh.cap = N;  // beyond the largest class
+    int B = 256, ipt = 4;
+    pcf_tuned_shape(PCF_KIND_AMER, mode, N, &B, &ipt);
+    if (o.block_size) B = o.block_size;
+    if (o.items_per_thread) ipt = o.items_per_thread;
+    sh.B = B;
+    sh.ipt = ipt;
+    sh.grid_paths = pcf_blocks_for(sh.cap / 2, B, 1);
+    sh.grid_step = pcf_blocks_for(sh.cap, B, ipt);
+    return sh;
+}
 
 struct LsmGraph {
     hipGraphExec_t exec = nullptr;
@@ -483,17 +506,66 @@ std::mutex& graph_mu() {
     return m;
 }
 
+// Builds the date sweep as an explicit graph (kernel nodes in a chain).  No
+// stream capture is involved, so other threads may use the device freely
+// while a graph is being built.
 template <typename Real, int SRC, bool REF>
-void enqueue_lsm(const LsmBufs& b, int M, int grid_paths, int grid_step, int B, hipStream_t s) {
+hipError_t build_lsm_graph(const LsmBufs& b, int M, int grid_paths, int grid_step, int B,
+                           hipGraph_t* out) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    if (e != hipSuccess) return e;
+    hipGraphNode_t prev = nullptr;
+    // kernel arguments are copied into the node at creation
+    LsmBufs bb = b;
+    std::vector<int> dates;
+    for (int m = M - 1; m >= 2; --m) dates.push_back(m);
+    auto add = [&](const void* fn, int grid, void** args) -> hipError_t {
+        hipKernelNodeParams p{};
+        p.func = const_cast<void*>(fn);
+        p.gridDim = dim3((unsigned)grid);
+        p.blockDim = dim3((unsigned)B);
+        p.sharedMemBytes = 0;
+        p.kernelParams = args;
+        p.extra = nullptr;
+        hipGraphNode_t n = nullptr;
+        hipError_t r = hipGraphAddKernelNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+        if (r == hipSuccess) prev = n;
+        return r;
+    };
+    void* a0[] = {&bb};
     if constexpr (SRC == kHost)
-        k_lsm_paths_host<REF><<<grid_paths, B, 0, s>>>(b);
+        e = add((const void*)k_lsm_paths_host<REF>, grid_paths, a0);
     else
-        k_lsm_paths_philox<Real, REF><<<grid_paths, B, 0, s>>>(b);
-    for (int m = M - 1; m >= 2; --m) k_lsm_step<Real, REF><<<grid_step, B, 0, s>>>(b, m);
-    k_lsm_final<Real, REF><<<grid_step, B, 0, s>>>(b);
+        e = add((const void*)k_lsm_paths_philox<Real, REF>, grid_paths, a0);
+    for (size_t i = 0; e == hipSuccess && i < dates.size(); ++i) {
+        void* as[] = {&bb, &dates[i]};
+        e = add((const void*)k_lsm_step<Real, REF>, grid_step, as);
+    }
+    if (e == hipSuccess) {
+        void* af[] = {&bb};
+        e = add((const void*)k_lsm_final<Real, REF>, grid_step, af);
+    }
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    *out = g;
+    return hipSuccess;
 }
 
 }  // namespace
+
+int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact) {
+    const bool f64 = ref_exact || rng == PCF_RNG_HOST || precision == PCF_FP64;
+    const int64_t rs = f64 ? 8 : 4;
+    pcf_opts o;
+    pcf_opts_init(&o);
+    const LsmShape sh = lsm_shape(N, f64 ? 1 : 0, o);
+    const int64_t grid_max = std::max(sh.grid_paths, sh.grid_step);
+    return (int64_t)M * sh.cap * rs + sh.cap * (4 + 2 * rs) + grid_max * 10 * 16 +
+           8 * (5 * (int64_t)(M + 1)) + 4096;
+}
 
 // Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
 void pcf_lsm_release(pcf_ctx* ctx) {
@@ -551,18 +623,18 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     P.M = M;
     P.seed = o.seed;
 
-    const size_t store_bytes = (size_t)M * (size_t)N * rs;
-    const size_t state_bytes = (size_t)N * (sizeof(int32_t) + 2 * rs);
+    const int mode = f64 ? 1 : 0;
+    const LsmShape sh = lsm_shape(N, mode, o);
+    const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
+    const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
     const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams);
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
 
-    const int B = o.block_size ? o.block_size : 256;
-    const int ipt = o.items_per_thread ? o.items_per_thread : 4;
-    const int grid_paths = (int)std::max<int64_t>(1, std::min<int64_t>((P.half + B - 1) / B, 1 << 20));
-    const int grid_step = (int)std::max<int64_t>(
-        1, std::min<int64_t>((N + (int64_t)B * ipt - 1) / ((int64_t)B * ipt), 1 << 20));
+    const int B = sh.B;
+    const int grid_paths = sh.grid_paths;
+    const int grid_step = sh.grid_step;
     const int grid_max = std::max(grid_paths, grid_step);
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid_max * 10 * sizeof(pcf::CSum)));
 
@@ -570,8 +642,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     b.store = ctx->lsm_store.ptr;
     char* sp = ctx->lsm_state.as<char>();
     b.when = (int32_t*)sp;
-    b.cf = sp + (size_t)N * sizeof(int32_t);
-    b.st = sp + (size_t)N * (sizeof(int32_t) + rs);
+    b.cf = sp + (size_t)sh.cap * sizeof(int32_t);
+    b.st = sp + (size_t)sh.cap * (sizeof(int32_t) + rs);
     double* cbase = ctx->lsm_coef.as<double>();
     b.coef = cbase;
     b.disc = cbase + 4 * (size_t)(M + 1);
@@ -592,7 +664,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
 
     // Captured graph per shape; re-captured if a buffer moved.
-    GraphKey key{N, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B};
+    GraphKey key{sh.cap, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B};
     hipGraphExec_t exec = nullptr;
     {
         std::lock_guard<std::mutex> gl(graph_mu());
@@ -604,20 +676,18 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g = LsmGraph{};
             hipGraph_t graph = nullptr;
-            PCF_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce;
             if (f64) {
-                if (src == kHost) {
-                    if (ref) enqueue_lsm<double, kHost, true>(b, M, grid_paths, grid_step, B, ctx->stream);
-                    else enqueue_lsm<double, kHost, false>(b, M, grid_paths, grid_step, B, ctx->stream);
-                } else {
-                    if (ref) enqueue_lsm<double, kPhilox, true>(b, M, grid_paths, grid_step, B, ctx->stream);
-                    else enqueue_lsm<double, kPhilox, false>(b, M, grid_paths, grid_step, B, ctx->stream);
-                }
+                if (src == kHost)
+                    ce = ref ? build_lsm_graph<double, kHost, true>(b, M, grid_paths, grid_step, B, &graph)
+                             : build_lsm_graph<double, kHost, false>(b, M, grid_paths, grid_step, B, &graph);
+                else
+                    ce = ref ? build_lsm_graph<double, kPhilox, true>(b, M, grid_paths, grid_step, B, &graph)
+                             : build_lsm_graph<double, kPhilox, false>(b, M, grid_paths, grid_step, B, &graph);
             } else {
-                enqueue_lsm<float, kPhilox, false>(b, M, grid_paths, grid_step, B, ctx->stream);
+                ce = build_lsm_graph<float, kPhilox, false>(b, M, grid_paths, grid_step, B, &graph);
             }
-            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-            if (ce != hipSuccess) return pcf_hip_fail(ctx, ce, "hipStreamEndCapture");
+            if (ce != hipSuccess) return pcf_hip_fail(ctx, ce, "building the LSM graph");
             ce = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (ce != hipSuccess) {
@@ -640,7 +710,12 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
 
     // Per-date flags.
     std::vector<double> coefs(4 * (size_t)(M + 1));
-    PCF_HIP_CHECK(ctx, hipMemcpy(coefs.data(), cbase, coefs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(coefs.data(), cbase, coefs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    if (o.when_out)
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                          ctx->stream));
+    PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int small = 0, fallback = 0, singular = 0;
     for (int m = 1; m <= M - 1; ++m) {
         const int f = (int)coefs[4 * m];
@@ -648,8 +723,6 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         fallback += f == kFlagReduced;
         singular += f == kFlagSingular;
     }
-    if (o.when_out)
-        PCF_HIP_CHECK(ctx, hipMemcpy(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost));
 
     const double n = (double)N;
     const double pay0 = std::max(payoff_fun * (S0 - E), 0.0);
@@ -668,6 +741,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     res->lsm_fallback = fallback;
     res->grid_blocks = grid_step;
     res->block_size = B;
+    res->batch_size = 1;
     // Algorithmic HBM bytes: store written once and read back once, per-path
     // state {cash flow, date} read + written per backward date.
     res->bytes = n * ((double)M * rs + (double)(M - 1) * rs + 2.0 * (M - 1) * (4.0 + rs));

@@ -199,18 +199,20 @@ __device__ __forceinline__ Acc<K> block_reduce(Acc<K> a, CSum* lds) {
 //   left zero on exit;  out: K doubles (value of each total sum).
 // Returns true in the (single) block that performed the final pass, whose
 // thread 0 then holds the totals in `tot`.
+//   blk / nblk: this block's index and the block count of the reduction (the
+//   launch grid for a single request; the request's share of a batched grid).
 template <int K>
 __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partials,
                                             unsigned int* __restrict__ counter, CSum* lds,
-                                            Acc<K>& tot) {
+                                            Acc<K>& tot, unsigned int blk, unsigned int nblk) {
     __shared__ bool is_last;
     a = block_reduce<K>(a, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = a.v[k];
+        for (int k = 0; k < K; ++k) partials[(size_t)blk * K + k] = a.v[k];
         __threadfence();
         const unsigned int prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
+        is_last = (prev == nblk - 1);
     }
     __syncthreads();
     if (!is_last) return false;
@@ -218,7 +220,7 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
     // Thread t folds partials t, t+B, t+2B, ... in order, then a block reduce.
     Acc<K> r;
     r.zero();
-    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    for (unsigned int b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < K; ++k) r.v[k] = csum_merge(r.v[k], partials[(size_t)b * K + k]);
     }
@@ -229,6 +231,37 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
         *counter = 0u;  // ready for the next launch
     }
     return true;
+}
+
+// Final FP64 moments of NP (sum, sum of squares) pairs: out[4k..4k+3] =
+// sum, sum of squares, mean, sample variance.
+template <int NP>
+__device__ __forceinline__ void write_moments(const Acc<2 * NP>& tot, int64_t N, double* out) {
+    const double n = (double)N;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const double sum = tot.v[2 * k].s + tot.v[2 * k].c;
+        const double sq = tot.v[2 * k + 1].s + tot.v[2 * k + 1].c;
+        const double mean = sum / n;
+        out[4 * k + 0] = sum;
+        out[4 * k + 1] = sq;
+        out[4 * k + 2] = mean;
+        out[4 * k + 3] = N > 1 ? (sq - sum * mean) / (n - 1.0) : 0.0;
+    }
+}
+
+// Batched launches: job j owns blocks [first_block, first_block + nblk) of the
+// grid.  Returns the job of block b (jobs sorted by first_block; b is
+// block-uniform, so the search runs on scalar loads).
+template <typename Job>
+__device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs, unsigned int b) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 }  // namespace pcf

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <unordered_map>
+#include <vector>
 
 #include "pcf.h"
 #include "pcf_device.hpp"
@@ -52,11 +54,21 @@ struct DevBuf {
     }
 };
 
-// Number of reduction slots (last-block counters) a context keeps; each
-// launch in a sequence that is in flight at once uses its own slot.
-constexpr int kCounterSlots = 64;
-// Max doubles of final results a kernel writes.
-constexpr int kOutDoubles = 64;
+// Requests per batched launch; each owns one last-block counter and
+// kOutPerJob doubles of results.
+constexpr int kMaxBatch = 64;
+constexpr int kCounterSlots = kMaxBatch;
+constexpr int kOutPerJob = 8;
+constexpr int kOutDoubles = kMaxBatch * kOutPerJob;
+// Bytes of one job record (launch descriptor) staged per batched launch.
+constexpr size_t kJobBytes = 512;
+
+// Device-resident basket constants (factor + per-asset terms), LRU slots.
+struct BasketCache {
+    std::unordered_map<uint64_t, int> where;
+    std::vector<uint64_t> key, stamp;
+    uint64_t clock = 0;
+};
 
 struct pcf_ctx {
     int device = 0;
@@ -81,7 +93,10 @@ struct pcf_ctx {
     DevBuf lsm_coef;   // mc_amer per-date coefficients / flags
     DevBuf basket_L;   // basket Cholesky factor + per-asset constants
     DevBuf scratch;    // misc (normals dump)
+    DevBuf jobs;       // job records of the current batched launch
+    BasketCache basket_cache;
     double* h_out = nullptr;  // pinned mirror of `out`
+    void* h_jobs = nullptr;   // pinned staging of `jobs`
 };
 
 // Releases the LSM graphs captured for a context (lsm.hip).
@@ -117,6 +132,59 @@ static inline pcf_opts pcf_opts_or_default(const pcf_opts* o) {
 }
 
 static inline int pcf_div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Blocks for `items` work items at B threads x ipt items per thread (>= 1).
+static inline int pcf_blocks_for(int64_t items, int B, int ipt) {
+    const int64_t per = (int64_t)B * (ipt > 0 ? ipt : 1);
+    int64_t b = (items + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > (1 << 20)) b = 1 << 20;
+    return (int)b;
+}
+
+// Size class of a request: ceil(log2 N) clamped to [10, 24].
+static inline int pcf_size_class(int64_t N) {
+    int c = 0;
+    while (((int64_t)1 << c) < N && c < 62) ++c;
+    return c < 10 ? 10 : (c > 24 ? 24 : c);
+}
+
+// Launch shape from the autotuner's table (autotune.cpp); leaves *B / *ipt
+// untouched when the (kind, mode, size class) entry was never tuned.
+void pcf_tuned_shape(int kind, int mode, int64_t N, int* B, int* ipt);
+
+static inline uint64_t pcf_make_batch_key(int kind, int mode, int variant, int B) {
+    return 1ull | ((uint64_t)(kind & 0xff) << 8) | ((uint64_t)(mode & 0xff) << 16) |
+           ((uint64_t)(variant & 0xffff) << 24) | ((uint64_t)(B & 0xffff) << 40);
+}
+
+static inline pcf_request pcf_make_request(int kind, double S0, double E, double r, double sigma,
+                                           double T, int64_t N, double pf, const pcf_opts* o) {
+    pcf_request q{};
+    q.kind = kind;
+    q.S0 = S0;
+    q.E = E;
+    q.r = r;
+    q.sigma = sigma;
+    q.T = T;
+    q.N = N;
+    q.payoff_fun = pf;
+    q.assets = 1;
+    q.opts = pcf_opts_or_default(o);
+    return q;
+}
+
+// Per-product batched runners: n requests sharing one batch key, one launch.
+// Per-request validation failures land in status[i]; the return value is a
+// launch-level failure.  *_batch_key returns 0 and sets *status on invalid input.
+uint64_t pcf_eur_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
+int pcf_eur_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
+uint64_t pcf_asia_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
+int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
+uint64_t pcf_basket_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
+int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
+// Device bytes an American request holds while in flight (path store + state).
+int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact);
 
 // glibc pow, never folded by the compiler.  Verification constants follow
 // the reference's -O0 build, which calls glibc pow(x, 2.0); compilers fold

@@ -57,3 +57,25 @@ def test_payoff_strings_like_reference():
         payoff_code("straddle")
     with pytest.raises(ValueError):
         payoff_code(0.5)
+
+
+def test_result_record_roundtrip():
+    """Every C result field lands in the Python Result (no GPU needed)."""
+    from parcompfin_amd.api import Result
+    r = N.PcfResult()
+    r.price, r.batch_size = 1.5, 3
+    out = Result.from_c(r, timing={"latency_ms": 1.0})
+    assert out.price == 1.5 and out.batch_size == 3 and out.extra["timing"]["latency_ms"] == 1.0
+
+
+def test_request_record_builds_without_device():
+    import ctypes as C
+    import numpy as np
+    from parcompfin_amd.api import Engine
+    o, keep = Engine.make_opts(seed=3, precision="fp64", normals=np.zeros(4), kernel_variant=1)
+    assert o.rng == N.PCF_RNG_HOST and o.n_host_normals == 4 and o.kernel_variant == 1
+    q = N.PcfRequest()
+    q.opts = o
+    ext = N.PcfBasketExt()
+    q.basket = C.pointer(ext)
+    assert bool(q.basket)
