@@ -208,6 +208,38 @@ def asia_roofline(eng, K=10):
             "flops_per_unit": "14 per path-step (normal 6, I update 4, S update 4)"}
 
 
+def profile_one(eng, which, K, W):
+    """One configuration at full size, W warm-up + K timed calls per precision."""
+    import numpy as np
+    sig8 = np.random.default_rng(8).uniform(0.15, 0.35, size=8)
+    sig32 = np.random.default_rng(32).uniform(0.15, 0.35, size=32)
+    calls = {
+        "cfg0": lambda p, s: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=s, precision=p),
+        "cfg1": lambda p, s: eng.binom_embar(100, 100, 0.05, 0.2, 1.0, 10**5, "call", ref_exact=(p == "fp64")),
+        "cfg2": lambda p, s: [eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3, sigmas=sg,
+                                               seed=s, precision=p) for d, sg in ((8, sig8), (32, sig32))][-1],
+        "cfg3": lambda p, s: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=s, precision=p),
+        "cfg4": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=s, precision=p),
+    }
+    out = {"profile": which}
+    if which == "mix":
+        from parcompfin_amd import Server
+        mix = make_mix(2026, 64)
+        with Server(eng, mem_budget_bytes=16 << 30, max_inflight=8) as srv:
+            for p in ("fp32k", "fp64"):
+                for _ in range(W):
+                    serve_once(srv, mix, p)
+                ts = [serve_once(srv, mix, p)[0] for _ in range(K)]
+                out[p] = {"ms_per_step": 1e3 * sum(ts) / K}
+        return out
+    for p in ("fp32k", "fp64"):
+        for k in range(W):
+            calls[which](p, 1 + k)
+        rs = [calls[which](p, 100 + k) for k in range(K)]
+        out[p] = {"kernel_ms": [r.kernel_ms for r in rs], "total_ms": [r.total_ms for r in rs]}
+    return out
+
+
 # ---------------------------------------------------------------- CPU baseline
 def cpu_baseline(mix, budget_s):
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -258,6 +290,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--profile", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "mix"],
+                    help="run only one configuration --steps times in FP32/Kahan then FP64 "
+                         "(for rocprofv3 kernel traces), print its timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: every box here has one GPU and no multi-GPU path runs in this "
@@ -269,6 +304,10 @@ def main():
     eng = Engine(0)
     info = eng.device_info()
     log("device:", info)
+    if args.profile:
+        print(json.dumps(profile_one(eng, args.profile, K, W)), flush=True)
+        eng.close()
+        return
     tuned = []
     if not args.no_autotune:
         t0 = time.perf_counter()

@@ -455,6 +455,319 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
     }
 }
 
+// --- persistent form ----------------------------------------------------------
+// One cooperative launch prices the whole request.  Thread t owns antithetic
+// pairs n = t, t + T, ... (T threads in the grid, at most PMAX pairs each) for
+// the forward path generation and the whole backward sweep, so the per-path
+// state (exercise date, true and booked cash flows) stays in registers and
+// each thread reads back only the store entries it wrote itself.  Per date,
+// one barrier-reduce: blocks publish their 8 partial sums, the last block to
+// arrive reduces them in a fixed order, solves, writes coef[m] and advances
+// the generation word every block waits on.  Every wait is bounded: past the
+// limit a block sets the error word and stops waiting (the host reports
+// PCF_ERR_TIMEOUT), so every wave always reaches the end of the kernel.
+struct LsmSync {
+    unsigned int* arrive;
+    unsigned int* gen;
+    unsigned int* err;
+};
+
+constexpr unsigned int kSpinLimit = 1u << 20;
+
+// Publish the block total (held by thread 0), count arrivals; true in the
+// last block, whose thread 0 then holds the fixed-order total in `tot`.
+// Coherent stores/loads + vmcnt ordering, no fences (see pcf_device.hpp).
+template <int K>
+__device__ __forceinline__ bool lsm_arrive(pcf::Acc<K> a, pcf::CSum* __restrict__ partials,
+                                           unsigned int* arrive, pcf::CSum* lds, pcf::Acc<K>& tot) {
+    __shared__ bool is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            pcf::publish(&partials[(size_t)blockIdx.x * K + k].s, a.v[k].s);
+            pcf::publish(&partials[(size_t)blockIdx.x * K + k].c, a.v[k].c);
+        }
+        pcf::stores_complete();
+        const unsigned int prev = atomicAdd(arrive, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    pcf::Acc<K> r;
+    r.zero();
+    for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            pcf::CSum p;
+            p.s = pcf::fetch(&partials[(size_t)bb * K + k].s);
+            p.c = pcf::fetch(&partials[(size_t)bb * K + k].c);
+            r.v[k] = pcf::csum_merge(r.v[k], p);
+        }
+    }
+    __syncthreads();
+    r = pcf::block_reduce<K>(r, lds);
+    if (threadIdx.x == 0) tot = r;
+    return true;
+}
+
+// coef[m] (published by the caller) and the reset arrival count are complete
+// before the generation word moves.
+__device__ __forceinline__ void lsm_release(const LsmSync& s, unsigned int target) {
+    pcf::publish_u32(s.arrive, 0u);
+    pcf::stores_complete();
+    pcf::publish_u32(s.gen, target);
+}
+
+// Wait (thread 0 spins, bounded) until generation `target` is released.
+// The poll is a coherent load; the only data read after the wait, coef[m],
+// is read with coherent loads and was complete before the generation moved.
+__device__ __forceinline__ void lsm_wait(const LsmSync& s, unsigned int target) {
+    if (threadIdx.x == 0) {
+        unsigned int it = 0;
+        while (pcf::fetch_u32(s.gen) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            // Past the bound, or once any block has given up, stop waiting.
+            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
+                atomicOr(s.err, 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Plain-FP64 block reduction of K sums in a fixed order (wave shuffles, then
+// waves in order); the result is valid in thread 0.
+template <int K>
+__device__ __forceinline__ void block_sum_plain(double v[K], double* lds) {
+#pragma unroll
+    for (int off = pcf::kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off, pcf::kWave);
+    const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + pcf::kWave - 1) / pcf::kWave;
+        for (int w = 1; w < nw; ++w)
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k] += lds[w * K + k];
+    }
+}
+
+template <typename Real, int SRC, bool REF, int PMAX>
+__global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) {
+    __shared__ pcf::CSum lds[16 * 8];
+    __shared__ double ldsp[16 * 8];
+    __shared__ double cm_sh[4];
+    const LsmParams& P = *b.P;
+    Real* __restrict__ store = (Real*)b.store;
+    const int64_t N = P.N, half = P.half;
+    const int M = P.M;
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Forward: the owned pairs' paths into the store (runtime loop: no state yet).
+    for (int64_t n = t0; n < half && n < t0 + (int64_t)PMAX * T; n += T) {
+        Real Sp = (Real)P.S0, Sm = (Real)P.S0;
+        if constexpr (SRC == kHost) {
+            for (int m = 1; m <= M; ++m) {
+#pragma clang fp contract(off)
+                const double w = b.hw[n * M + (m - 1)];
+                Sp = Sp * exp(P.h_mu + P.sigma * w);
+                Sm = Sm * exp(P.h_mu - P.sigma * w);
+                store[(int64_t)(m - 1) * N + n] = Sp;
+                store[(int64_t)(m - 1) * N + n + half] = Sm;
+            }
+        } else {
+            for (int m0 = 1; m0 <= M; m0 += 4) {
+                Real z[4];
+                pcf::normals4<Real>(P.seed, PCF_STREAM_AMER, (uint64_t)n, (uint32_t)((m0 - 1) >> 2), z);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int m = m0 + q;
+                    if (m <= M) {
+                        if constexpr (sizeof(Real) == 4) {
+                            Sp *= pcf::exp2_f32(fmaf(P.vol2f, z[q], P.mu2f));
+                            Sm *= pcf::exp2_f32(fmaf(-P.vol2f, z[q], P.mu2f));
+                        } else {
+                            Sp *= exp(fma(P.vol, (double)z[q], P.mu));
+                            Sm *= exp(fma(-P.vol, (double)z[q], P.mu));
+                        }
+                        store[(int64_t)(m - 1) * N + n] = Sp;
+                        store[(int64_t)(m - 1) * N + n + half] = Sm;
+                    }
+                }
+            }
+        }
+    }
+    // State of the owned paths (slot 2k: path n, 2k+1: path n + N/2), from the
+    // entries this thread just wrote.
+    int32_t when[2 * PMAX];
+    Real cf[2 * PMAX];
+    Real stb[REF ? 2 * PMAX : 1];
+    double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        const int64_t n = t0 + (int64_t)k * T;
+        when[2 * k] = when[2 * k + 1] = M;
+        cf[2 * k] = cf[2 * k + 1] = Real(0);
+        if (n < half) {
+            cf[2 * k] = pay<Real>(store[(int64_t)(M - 1) * N + n], (Real)P.E, (Real)P.pf);
+            cf[2 * k + 1] = pay<Real>(store[(int64_t)(M - 1) * N + n + half], (Real)P.E, (Real)P.pf);
+            e0 += (double)cf[2 * k] + (double)cf[2 * k + 1];
+            e1 += (double)cf[2 * k] * (double)cf[2 * k] + (double)cf[2 * k + 1] * (double)cf[2 * k + 1];
+        }
+        if constexpr (REF) {
+            stb[2 * k] = cf[2 * k];
+            stb[2 * k + 1] = cf[2 * k + 1];
+        }
+    }
+    // Backward sweep: one barrier-reduce per exercise date.
+    unsigned int gen = 0;
+    for (int m = M - 1; m >= 1; --m) {
+        ++gen;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            const int64_t n = t0 + (int64_t)k * T;
+            if (n < half) {
+                lsm_accumulate<Real, REF>(P, b.disc, m, store[(int64_t)(m - 1) * N + n], when[2 * k], cf[2 * k], acc);
+                lsm_accumulate<Real, REF>(P, b.disc, m, store[(int64_t)(m - 1) * N + n + half], when[2 * k + 1],
+                                          cf[2 * k + 1], acc);
+            }
+        }
+        block_sum_plain<8>(acc, ldsp);
+        pcf::Acc<8> a, tot;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a.v[k] = {acc[k], 0.0};
+        // a is meaningful in thread 0 only; block_reduce below adds zeros elsewhere.
+        if (threadIdx.x != 0)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a.v[k] = {0.0, 0.0};
+        if (lsm_arrive<8>(a, b.partials, sy.arrive, lds, tot)) {
+            if (threadIdx.x == 0) {
+                lsm_solve<REF>(tot, b.coef, m);
+                double* c = b.coef + 4 * m;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    cm_sh[k] = c[k];
+                    pcf::publish(c + k, cm_sh[k]);
+                }
+                lsm_release(sy, gen);
+            }
+            __syncthreads();
+        } else {
+            lsm_wait(sy, gen);
+            if (threadIdx.x == 0) {
+                const double* c = b.coef + 4 * m;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) cm_sh[k] = pcf::fetch(c + k);
+            }
+            __syncthreads();
+        }
+        const double cm[4] = {cm_sh[0], cm_sh[1], cm_sh[2], cm_sh[3]};
+        if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
+#pragma unroll
+            for (int k = 0; k < PMAX; ++k) {
+                const int64_t n = t0 + (int64_t)k * T;
+                if (n < half) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int i = 2 * k + h;
+                        Real stv = REF ? stb[REF ? i : 0] : cf[i];
+                        (void)lsm_decide<Real, REF>(P, b.disc, cm, m, store[(int64_t)(m - 1) * N + n + h * half],
+                                                    when[i], cf[i], stv);
+                        if constexpr (REF) stb[i] = stv;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // cm_sh is rewritten at the next date
+    }
+    // Final: discounted booked cash flows + European control.
+    double s = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        const int64_t n = t0 + (int64_t)k * T;
+        if (n < half) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = 2 * k + h;
+                double v;
+                if constexpr (REF) {
+#pragma clang fp contract(off)
+                    v = ((double)stb[i] != 0.0) ? exp(P.nr * (double)when[i] * P.dt) * (double)stb[i] : 0.0;
+                } else {
+                    v = b.disc[when[i]] * (double)cf[i];
+                }
+                s += v;
+                s2 += v * v;
+                b.when[n + h * half] = when[i];
+            }
+        }
+    }
+    double fin[4] = {e0, e1, s, s2};
+    block_sum_plain<4>(fin, ldsp);
+    pcf::Acc<4> a, tot;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.v[k] = {threadIdx.x == 0 ? fin[k] : 0.0, 0.0};
+    if (lsm_arrive<4>(a, b.partials, sy.arrive, lds, tot) && threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) b.out[k] = tot.v[k].s + tot.v[k].c;
+        pcf::publish_u32(sy.arrive, 0u);
+        pcf::publish_u32(sy.gen, 0u);
+    }
+}
+
+template <typename Real, int SRC, bool REF, int PMAX>
+hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, int B, hipStream_t s) {
+    LsmBufs bb = b;
+    LsmSync ss = sy;
+    void* args[] = {&bb, &ss};
+    return hipLaunchCooperativeKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
+                                      dim3((unsigned)B), args, 0, s);
+}
+
+template <typename Real, int SRC, bool REF, int PMAX>
+int max_coresident_t(int B, int cus) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, B,
+                                                     0) != hipSuccess)
+        return 0;
+    return nb * cus;
+}
+
+// Dispatch over (Real, SRC, REF) and the register budget PMAX in {1,2,4,8}.
+template <int PMAX>
+hipError_t launch_persistent(bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid, int B,
+                             hipStream_t s) {
+    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX>(b, sy, grid, B, s);
+    if (src == kHost)
+        return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s)
+                   : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s);
+    return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s)
+               : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s);
+}
+
+template <int PMAX>
+int max_coresident(bool f64, int src, bool ref, int B, int cus) {
+    if (!f64) return max_coresident_t<float, kPhilox, false, PMAX>(B, cus);
+    if (src == kHost)
+        return ref ? max_coresident_t<double, kHost, true, PMAX>(B, cus)
+                   : max_coresident_t<double, kHost, false, PMAX>(B, cus);
+    return ref ? max_coresident_t<double, kPhilox, true, PMAX>(B, cus)
+               : max_coresident_t<double, kPhilox, false, PMAX>(B, cus);
+}
+
+// One persistent LSM kernel on the device at a time (two of them could each
+// hold part of the CUs while waiting for their remaining blocks).
+std::mutex& persistent_mu() {
+    static std::mutex m;
+    return m;
+}
+
 // ---- host: buffers, graph cache --------------------------------------------
 // Graph shapes depend on the size class of N (capacity 2^ceil(log2 N)), not
 // on N itself: the kernels read N from the parameter block, so any N of a
@@ -627,7 +940,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const LsmShape sh = lsm_shape(N, mode, o);
     const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
-    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams);
+    // coef (4 per date) | discount table | parameter block | sync words
+    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) + 64;
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
@@ -663,10 +977,59 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
                                       hipMemcpyHostToDevice, ctx->stream));
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
 
-    // Captured graph per shape; re-captured if a buffer moved.
+    unsigned int* sync_words = (unsigned int*)((char*)Pdev + ((sizeof(LsmParams) + 15) & ~(size_t)15));
+
+    // Persistent form when the request fits the register budget of a
+    // co-resident grid (pairs per thread PMAX in {1,2,4,8}, starting from the
+    // tuned items per thread); otherwise the graph of per-date launches.
+    bool persistent = false;
+    int pgrid = 0, pmax = 0;
+    // 1024-thread blocks: the per-date barrier costs one arrival per block.
+    const int PB = 1024;
+    if (o.kernel_variant != 1) {
+        int want = 1;
+        while (want < sh.ipt && want < 8) want <<= 1;
+        for (int pm = want; pm <= 8 && !persistent; pm <<= 1) {
+            int gmax = 0;
+            if (pm == 1) gmax = max_coresident<1>(f64, src, ref, PB, ctx->cu_count);
+            else if (pm == 2) gmax = max_coresident<2>(f64, src, ref, PB, ctx->cu_count);
+            else if (pm == 4) gmax = max_coresident<4>(f64, src, ref, PB, ctx->cu_count);
+            else gmax = max_coresident<8>(f64, src, ref, PB, ctx->cu_count);
+            const int64_t g = (P.half + (int64_t)pm * PB - 1) / ((int64_t)pm * PB);
+            if (g >= 1 && g <= gmax) {
+                persistent = true;
+                pgrid = (int)g;
+                pmax = pm;
+            }
+        }
+        if (o.kernel_variant == 2 && !persistent)
+            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "persistent LSM form does not fit N=%lld", (long long)N);
+    }
+    if (persistent) {
+        PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * 8 * sizeof(pcf::CSum)));
+        b.partials = ctx->partials.as<pcf::CSum>();
+        LsmSync sy{sync_words, sync_words + 1, sync_words + 2};
+        PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, 16, ctx->stream));
+        {
+            std::lock_guard<std::mutex> pl(persistent_mu());
+            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+            hipError_t le;
+            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
+            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
+            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
+            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
+            if (le != hipSuccess) return pcf_hip_fail(ctx, le, "hipLaunchCooperativeKernel (LSM)");
+            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+            if ((st = pcf_timing_end(ctx, 4, res))) return st;
+        }
+        res->grid_blocks = pgrid;
+        res->block_size = PB;
+    }
+
+    // Graph per shape; rebuilt if a buffer moved.
     GraphKey key{sh.cap, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B};
     hipGraphExec_t exec = nullptr;
-    {
+    if (!persistent) {
         std::lock_guard<std::mutex> gl(graph_mu());
         auto& g = graph_cache()[ctx][key];
         const bool fresh = g.exec && g.store_ptr == b.store && g.state_ptr == (void*)b.when &&
@@ -702,10 +1065,14 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         }
         exec = g.exec;
     }
-    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
-    PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
-    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
-    if ((st = pcf_timing_end(ctx, 4, res))) return st;
+    if (!persistent) {
+        PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+        if ((st = pcf_timing_end(ctx, 4, res))) return st;
+        res->grid_blocks = grid_step;
+        res->block_size = B;
+    }
     const double h0 = ctx->h_out[0], h1 = ctx->h_out[1], h2 = ctx->h_out[2], h3 = ctx->h_out[3];
 
     // Per-date flags.
@@ -715,7 +1082,12 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     if (o.when_out)
         PCF_HIP_CHECK(ctx, hipMemcpyAsync(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost,
                                           ctx->stream));
+    unsigned int sync_host[4] = {0, 0, 0, 0};
+    if (persistent)
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_host, sync_words, sizeof sync_host, hipMemcpyDeviceToHost, ctx->stream));
     PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (sync_host[2])
+        return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
     int small = 0, fallback = 0, singular = 0;
     for (int m = 1; m <= M - 1; ++m) {
         const int f = (int)coefs[4 * m];
@@ -739,8 +1111,6 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     res->n_steps = M;
     res->lsm_small_itm = small;
     res->lsm_fallback = fallback;
-    res->grid_blocks = grid_step;
-    res->block_size = B;
     res->batch_size = 1;
     // Algorithmic HBM bytes: store written once and read back once, per-path
     // state {cash flow, date} read + written per backward date.

@@ -194,11 +194,30 @@ __device__ __forceinline__ Acc<K> block_reduce(Acc<K> a, CSum* lds) {
     return a;
 }
 
+// Cross-block data (partials, counters, flags) is published with coherent
+// agent-scope atomic stores, which write through the per-XCD L2, and read
+// with coherent atomic loads.  Ordering a block's published values before its
+// arrival only needs those stores to be complete (vmcnt 0), so no agent-scope
+// fence -- i.e. no whole-L2 write-back or invalidate -- is needed anywhere.
+__device__ __forceinline__ void publish(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double fetch(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish_u32(unsigned int* p, unsigned int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int fetch_u32(const unsigned int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until every memory operation this wave issued is complete.
+__device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+
 // Per-block partial write + "last block finishes" grid reduction.
-//   partials: gridDim.x * K CSums;  counter: one uint32, zero at launch and
-//   left zero on exit;  out: K doubles (value of each total sum).
-// Returns true in the (single) block that performed the final pass, whose
-// thread 0 then holds the totals in `tot`.
+//   partials: nblk * K CSums;  counter: one uint32, zero at launch and left
+//   zero on exit.  Returns true in the (single) block that performed the
+//   final pass, whose thread 0 then holds the totals in `tot`.
 //   blk / nblk: this block's index and the block count of the reduction (the
 //   launch grid for a single request; the request's share of a batched grid).
 template <int K>
@@ -209,26 +228,33 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
     a = block_reduce<K>(a, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) partials[(size_t)blk * K + k] = a.v[k];
-        __threadfence();
+        for (int k = 0; k < K; ++k) {
+            publish(&partials[(size_t)blk * K + k].s, a.v[k].s);
+            publish(&partials[(size_t)blk * K + k].c, a.v[k].c);
+        }
+        stores_complete();
         const unsigned int prev = atomicAdd(counter, 1u);
         is_last = (prev == nblk - 1);
     }
     __syncthreads();
     if (!is_last) return false;
-    __threadfence();
     // Thread t folds partials t, t+B, t+2B, ... in order, then a block reduce.
     Acc<K> r;
     r.zero();
     for (unsigned int b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) r.v[k] = csum_merge(r.v[k], partials[(size_t)b * K + k]);
+        for (int k = 0; k < K; ++k) {
+            CSum p;
+            p.s = fetch(&partials[(size_t)b * K + k].s);
+            p.c = fetch(&partials[(size_t)b * K + k].c);
+            r.v[k] = csum_merge(r.v[k], p);
+        }
     }
     __syncthreads();  // lds reuse
     r = block_reduce<K>(r, lds);
     if (threadIdx.x == 0) {
         tot = r;
-        *counter = 0u;  // ready for the next launch
+        publish_u32(counter, 0u);  // ready for the next launch
     }
     return true;
 }

@@ -235,3 +235,24 @@ def test_amer_errors_like_reference(engine):
     # production falls back to a reduced basis instead
     r = engine.mc_amer(100, 110, 0.01, 0.0, 1.0, 1024, 10, "put", normals=w)
     assert r.lsm_fallback > 0
+
+
+@pytest.mark.parametrize("N,M,prec,ref", [(1 << 16, 50, "fp32k", False), (1 << 16, 50, "fp64", False),
+                                          (4096, 25, "fp64", True), (1 << 20, 50, "fp32k", False),
+                                          (1 << 22, 50, "fp32k", False), (2, 3, "fp64", False)])
+def test_amer_persistent_matches_graph_form(engine, N, M, prec, ref):
+    """The one-launch persistent sweep and the per-date graph give the same
+    exercise dates (up to a flipped decision at the rounding boundary) and
+    prices."""
+    say("amer persistent vs graph", N, M, prec, ref)
+    kw = dict(seed=29, precision=prec, ref_exact=ref, return_when=True)
+    g = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", kernel_variant=1, **kw)
+    p = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", kernel_variant=2, **kw)
+    flips = int((g.extra["when"] != p.extra["when"]).sum())
+    say("  flips", flips, "grid", p.grid_blocks, "kernel_ms graph/persistent", g.kernel_ms, p.kernel_ms)
+    assert flips <= max(2, N // 100000)
+    if flips == 0:
+        assert abs(g.price - p.price) <= 1e-12 * g.price
+    else:
+        assert abs(g.price - p.price) <= 0.01 * g.std_err
+    assert g.control_price == p.control_price or abs(g.control_price - p.control_price) < 1e-12 * g.control_price
