@@ -82,7 +82,11 @@ typedef struct pcf_opts {
     int32_t items_per_thread;/* paths (or path groups) per thread; 0 = tuned default */
     int32_t* when_out;       /* mc_amer: optional host array of N exercise dates */
     int32_t kernel_variant;  /* 0 = automatic; mc_eur_multi FP32: 1 = VALU FMA kernel,
-                                2 = MFMA kernel (needs 16 <= assets <= 32) */
+                                2 = MFMA kernel (needs 16 <= assets <= 32);
+                                mc_amer: 1 = graph of per-date launches, 2 = one
+                                persistent launch (error if its grid cannot be
+                                co-resident; block_size / items_per_thread, when
+                                set, give its block size and pairs per thread) */
     int32_t reserved[5];
 } pcf_opts;
 

@@ -58,6 +58,7 @@ double time_shape(pcf_ctx* ctx, int kind, int64_t N, int assets, int B, int ipt,
                 st = pcf_mc_eur_multi(ctx, 100, 100, 0.05, 0.25, 1.0, N, 1.0, assets, 0.3, nullptr, &o, &res);
                 break;
             default:
+                o.kernel_variant = 2;  // the tuned shape is the persistent form's
                 st = pcf_mc_amer(ctx, 100, 100, 0.06, 0.2, 1.0, N & ~(int64_t)1, 50, -1.0, &o, &res);
                 break;
         }
@@ -114,7 +115,8 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
         targets.push_back({PCF_KIND_BASKET, 1, 32, grid({64, 128, 256}, {1, 2, 4, 8}), {256, 2}});
     }
     if (kind_mask & (1 << PCF_KIND_AMER))
-        targets.push_back({PCF_KIND_AMER, 0, 1, grid({128, 256}, {1, 2, 4, 8}), {256, 4}});
+        // persistent LSM: (block size, pairs per thread); {0, 0} = built-in choice
+        targets.push_back({PCF_KIND_AMER, 0, 1, grid({64, 128, 256, 512, 1024}, {1, 2, 4, 8}), {0, 0}});
 
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     int nout = 0;
@@ -123,7 +125,12 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
         for (int c = 14; c <= maxc; c += 2) {
             const int64_t N = (int64_t)1 << c;
             const int reps = c <= 18 ? 4 : 2;
+            {
+                std::lock_guard<std::mutex> tl(tmu());
+                table()[t.kind][t.mode][c] = Shape{};  // the default is measured untuned
+            }
             const double def_ms = time_shape(ctx, t.kind, N, t.assets, t.def.B, t.def.ipt, reps);
+            if (def_ms < 0 && t.kind == PCF_KIND_AMER) continue;  // beyond the persistent form
             if (def_ms < 0) return pcf_fail(ctx, PCF_ERR_HIP, "autotune: default shape failed: %s",
                                             ctx->last_error.c_str());
             Candidate best = t.def;
@@ -151,12 +158,16 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
                 out[nout] = pcf_tune_entry{t.kind, t.mode, c, best.B, best.ipt, ncand, best_ms, def_ms};
             ++nout;
         }
-        // Unmeasured classes take the nearest measured one.
+        // Unmeasured classes take the nearest measured one (for the persistent
+        // LSM shape only inside the measured range: beyond it the built-in
+        // choice adapts to the size and the graph form takes over).
+        if (measured.empty()) continue;
         std::lock_guard<std::mutex> tl(tmu());
         for (int c = 10; c < kClasses; ++c) {
             int nearest = measured.front();
             for (int m : measured)
                 if (std::abs(m - c) < std::abs(nearest - c)) nearest = m;
+            if (t.kind == PCF_KIND_AMER && (c < measured.front() || c > measured.back())) continue;
             if (std::find(measured.begin(), measured.end(), c) == measured.end())
                 table()[t.kind][t.mode][c] = table()[t.kind][t.mode][nearest];
         }

@@ -29,6 +29,8 @@
 //    exercise test payoff(S, E); FP32 or FP64 path store.
 // Dates with 1-2 in-the-money paths compare payoff with the discounted cash
 // flow directly (src/mc_amer.cpp:75-83), in both forms.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -461,15 +463,20 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
 // the forward path generation and the whole backward sweep, so the per-path
 // state (exercise date, true and booked cash flows) stays in registers and
 // each thread reads back only the store entries it wrote itself.  Per date,
-// one barrier-reduce: blocks publish their 8 partial sums, the last block to
-// arrive reduces them in a fixed order, solves, writes coef[m] and advances
-// the generation word every block waits on.  Every wait is bounded: past the
-// limit a block sets the error word and stops waiting (the host reports
-// PCF_ERR_TIMEOUT), so every wave always reaches the end of the kernel.
+// one all-reduce: every block publishes its 8 sums and counts its arrival on
+// a counter that only grows (date k of the sweep is complete at k * grid
+// arrivals); once all blocks have arrived, every block sums the partials in
+// the same fixed order and solves the same normal equations itself, so no
+// block waits for another's solve.  Partials alternate between two slot sets
+// by date parity: a block can write date m only after every block arrived at
+// date m+1, that is after every block finished reading date m+2's slots.
+// Every wait is bounded: past the limit a block sets the error word and stops
+// waiting (the host reports PCF_ERR_TIMEOUT), so every wave always reaches
+// the end of the kernel.
 struct LsmSync {
-    unsigned int* arrive;
-    unsigned int* gen;
-    unsigned int* err;
+    unsigned int* arrive;  // per-date arrivals, growing by grid each date
+    unsigned int* fin;     // arrivals at the final sums
+    unsigned int* err;     // nonzero once any wait gave up
 };
 
 constexpr unsigned int kSpinLimit = 1u << 20;
@@ -510,32 +517,6 @@ __device__ __forceinline__ bool lsm_arrive(pcf::Acc<K> a, pcf::CSum* __restrict_
     return true;
 }
 
-// coef[m] (published by the caller) and the reset arrival count are complete
-// before the generation word moves.
-__device__ __forceinline__ void lsm_release(const LsmSync& s, unsigned int target) {
-    pcf::publish_u32(s.arrive, 0u);
-    pcf::stores_complete();
-    pcf::publish_u32(s.gen, target);
-}
-
-// Wait (thread 0 spins, bounded) until generation `target` is released.
-// The poll is a coherent load; the only data read after the wait, coef[m],
-// is read with coherent loads and was complete before the generation moved.
-__device__ __forceinline__ void lsm_wait(const LsmSync& s, unsigned int target) {
-    if (threadIdx.x == 0) {
-        unsigned int it = 0;
-        while (pcf::fetch_u32(s.gen) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            // Past the bound, or once any block has given up, stop waiting.
-            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
-                atomicOr(s.err, 1u);
-                break;
-            }
-        }
-    }
-    __syncthreads();
-}
-
 // Plain-FP64 block reduction of K sums in a fixed order (wave shuffles, then
 // waves in order); the result is valid in thread 0.
 template <int K>
@@ -557,17 +538,64 @@ __device__ __forceinline__ void block_sum_plain(double v[K], double* lds) {
     }
 }
 
+// Per-date all-reduce of the 8 regression sums.  On entry thread 0 holds the
+// block's sums in v; on return thread 0 holds the grid totals, identical in
+// every block.  `slots` holds one row of 8 per block for this date's parity;
+// `target` = arrivals that complete this date.  The partials are published
+// with coherent stores that complete before the arrival is counted, and read
+// with coherent loads after the count is seen complete.
+template <int K>
+__device__ __forceinline__ void lsm_allreduce(double v[K], double* __restrict__ slots, const LsmSync& s,
+                                              unsigned int target, double* ldsp) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) pcf::publish(slots + (size_t)blockIdx.x * K + k, v[k]);
+        pcf::stores_complete();
+        __hip_atomic_fetch_add(s.arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned int it = 0;
+        while (pcf::fetch_u32(s.arrive) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            // Past the bound, or once any block has given up, stop waiting.
+            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
+                atomicOr(s.err, 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    double t[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] = 0.0;
+    for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; ++k) t[k] += pcf::fetch(slots + (size_t)bb * K + k);
+    block_sum_plain<K>(t, ldsp);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = t[k];
+}
+
+// Discount factors exp(-r dt k) served from LDS when M + 1 fits.
+constexpr int kDiscLds = 1024;
+
 template <typename Real, int SRC, bool REF, int PMAX>
 __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) {
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
     __shared__ double cm_sh[4];
+    __shared__ double disc_sh[kDiscLds];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool disc_in_lds = M + 1 <= kDiscLds;
+    if (disc_in_lds)
+        for (int k = threadIdx.x; k <= M; k += blockDim.x) disc_sh[k] = b.disc[k];
+    const double* __restrict__ disc = disc_in_lds ? disc_sh : b.disc;
+    // Date slots: 2 parities x grid x 8 doubles, then the final sums' slots.
+    double* __restrict__ slots = (double*)b.partials;
+    __syncthreads();
     // Forward: the owned pairs' paths into the store (runtime loop: no state yet).
     for (int64_t n = t0; n < half && n < t0 + (int64_t)PMAX * T; n += T) {
         Real Sp = (Real)P.S0, Sm = (Real)P.S0;
@@ -624,49 +652,49 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             stb[2 * k + 1] = cf[2 * k + 1];
         }
     }
-    // Backward sweep: one barrier-reduce per exercise date.
+    // Backward sweep: one barrier-reduce per exercise date.  Sc holds the
+    // owned paths' values at date m; the next date's row is loaded into Sn
+    // before the barrier so its latency hides behind the wait.
+    // (no prefetch at 8 pairs per thread: its registers would spill)
+    constexpr bool kPrefetch = PMAX <= 4;
+    Real Sc[2 * PMAX], Sn[kPrefetch ? 2 * PMAX : 1];
+    auto load_row = [&](int m, Real* dst) {
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            const int64_t n = t0 + (int64_t)k * T;
+            const bool own = n < half;
+            dst[2 * k] = own ? store[(int64_t)(m - 1) * N + n] : Real(0);
+            dst[2 * k + 1] = own ? store[(int64_t)(m - 1) * N + n + half] : Real(0);
+        }
+    };
+    if (kPrefetch && M >= 2) load_row(M - 1, Sc);
     unsigned int gen = 0;
     for (int m = M - 1; m >= 1; --m) {
         ++gen;
+        if constexpr (!kPrefetch) load_row(m, Sc);
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
             const int64_t n = t0 + (int64_t)k * T;
             if (n < half) {
-                lsm_accumulate<Real, REF>(P, b.disc, m, store[(int64_t)(m - 1) * N + n], when[2 * k], cf[2 * k], acc);
-                lsm_accumulate<Real, REF>(P, b.disc, m, store[(int64_t)(m - 1) * N + n + half], when[2 * k + 1],
-                                          cf[2 * k + 1], acc);
+                lsm_accumulate<Real, REF>(P, disc, m, Sc[2 * k], when[2 * k], cf[2 * k], acc);
+                lsm_accumulate<Real, REF>(P, disc, m, Sc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
             }
         }
+        if constexpr (kPrefetch)
+            if (m > 1) load_row(m - 1, Sn);
         block_sum_plain<8>(acc, ldsp);
-        pcf::Acc<8> a, tot;
+        lsm_allreduce<8>(acc, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen * gridDim.x, ldsp);
+        if (threadIdx.x == 0) {
+            pcf::Acc<8> tot;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) a.v[k] = {acc[k], 0.0};
-        // a is meaningful in thread 0 only; block_reduce below adds zeros elsewhere.
-        if (threadIdx.x != 0)
+            for (int k = 0; k < 8; ++k) tot.v[k] = {acc[k], 0.0};
+            lsm_solve<REF>(tot, cm_sh, 0);
+            if (blockIdx.x == 0)
 #pragma unroll
-            for (int k = 0; k < 8; ++k) a.v[k] = {0.0, 0.0};
-        if (lsm_arrive<8>(a, b.partials, sy.arrive, lds, tot)) {
-            if (threadIdx.x == 0) {
-                lsm_solve<REF>(tot, b.coef, m);
-                double* c = b.coef + 4 * m;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    cm_sh[k] = c[k];
-                    pcf::publish(c + k, cm_sh[k]);
-                }
-                lsm_release(sy, gen);
-            }
-            __syncthreads();
-        } else {
-            lsm_wait(sy, gen);
-            if (threadIdx.x == 0) {
-                const double* c = b.coef + 4 * m;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) cm_sh[k] = pcf::fetch(c + k);
-            }
-            __syncthreads();
+                for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm_sh[k];
         }
+        __syncthreads();
         const double cm[4] = {cm_sh[0], cm_sh[1], cm_sh[2], cm_sh[3]};
         if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
 #pragma unroll
@@ -677,13 +705,15 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
                     for (int h = 0; h < 2; ++h) {
                         const int i = 2 * k + h;
                         Real stv = REF ? stb[REF ? i : 0] : cf[i];
-                        (void)lsm_decide<Real, REF>(P, b.disc, cm, m, store[(int64_t)(m - 1) * N + n + h * half],
-                                                    when[i], cf[i], stv);
+                        (void)lsm_decide<Real, REF>(P, disc, cm, m, Sc[i], when[i], cf[i], stv);
                         if constexpr (REF) stb[i] = stv;
                     }
                 }
             }
         }
+        if constexpr (kPrefetch)
+#pragma unroll
+            for (int i = 0; i < 2 * PMAX; ++i) Sc[i] = Sn[i];
         __syncthreads();  // cm_sh is rewritten at the next date
     }
     // Final: discounted booked cash flows + European control.
@@ -700,7 +730,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
 #pragma clang fp contract(off)
                     v = ((double)stb[i] != 0.0) ? exp(P.nr * (double)when[i] * P.dt) * (double)stb[i] : 0.0;
                 } else {
-                    v = b.disc[when[i]] * (double)cf[i];
+                    v = disc[when[i]] * (double)cf[i];
                 }
                 s += v;
                 s2 += v * v;
@@ -713,11 +743,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     pcf::Acc<4> a, tot;
 #pragma unroll
     for (int k = 0; k < 4; ++k) a.v[k] = {threadIdx.x == 0 ? fin[k] : 0.0, 0.0};
-    if (lsm_arrive<4>(a, b.partials, sy.arrive, lds, tot) && threadIdx.x == 0) {
+    // Final sums: last-arriving block, on its own counter and slots.
+    if (lsm_arrive<4>(a, (pcf::CSum*)(slots + (size_t)2 * gridDim.x * 8), sy.fin, lds, tot) && threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) b.out[k] = tot.v[k].s + tot.v[k].c;
-        pcf::publish_u32(sy.arrive, 0u);
-        pcf::publish_u32(sy.gen, 0u);
     }
 }
 
@@ -790,8 +819,8 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o) {
     LsmShape sh;
     sh.cap = (int64_t)1 << pcf_size_class(N);
     if (sh.cap < N) sh.cap = N;  // beyond the largest class
-    int B = 256, ipt = 4;
-    pcf_tuned_shape(PCF_KIND_AMER, mode, N, &B, &ipt);
+    (void)mode;
+    int B = 256, ipt = 4;  // the tuned table holds the persistent form's shape
     if (o.block_size) B = o.block_size;
     if (o.items_per_thread) ipt = o.items_per_thread;
     sh.B = B;
@@ -799,6 +828,24 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o) {
     sh.grid_paths = pcf_blocks_for(sh.cap / 2, B, 1);
     sh.grid_step = pcf_blocks_for(sh.cap, B, ipt);
     return sh;
+}
+
+// Default shape of the persistent form.  Per date every block reads every
+// block's partials (grid^2 coherent loads) and each CU works through its own
+// pairs, so the grid targets 64 blocks for up to 2^13 pairs, 128 up to 2^17
+// and one per CU beyond; pairs per block are then split into at most 4 per
+// thread (8 only when a 1024-thread block is not enough).
+void lsm_persistent_shape(int64_t half, int cus, int* bs, int* pm) {
+    const int64_t target = half <= (1 << 13) ? 64 : half <= (1 << 17) ? 128 : std::max(cus, 64);
+    const int64_t per_block = (half + target - 1) / target;
+    int p = (int)std::min<int64_t>(4, std::max<int64_t>(1, per_block / 256));
+    int64_t b = ((per_block + p - 1) / p + 63) / 64 * 64;
+    if (b > 1024) {
+        p = 8;
+        b = ((per_block + 7) / 8 + 63) / 64 * 64;
+    }
+    *pm = p;
+    *bs = (int)std::min<int64_t>(1024, std::max<int64_t>(64, b));
 }
 
 struct LsmGraph {
@@ -876,8 +923,9 @@ int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exa
     pcf_opts_init(&o);
     const LsmShape sh = lsm_shape(N, f64 ? 1 : 0, o);
     const int64_t grid_max = std::max(sh.grid_paths, sh.grid_step);
+    // + the persistent form's partials: at most 4096 co-resident blocks x 192 B
     return (int64_t)M * sh.cap * rs + sh.cap * (4 + 2 * rs) + grid_max * 10 * 16 +
-           8 * (5 * (int64_t)(M + 1)) + 4096;
+           8 * (5 * (int64_t)(M + 1)) + 4096 + 4096 * 192;
 }
 
 // Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
@@ -979,34 +1027,47 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
 
     unsigned int* sync_words = (unsigned int*)((char*)Pdev + ((sizeof(LsmParams) + 15) & ~(size_t)15));
 
-    // Persistent form when the request fits the register budget of a
-    // co-resident grid (pairs per thread PMAX in {1,2,4,8}, starting from the
-    // tuned items per thread); otherwise the graph of per-date launches.
+    // Persistent form when the request fits a co-resident grid.  Shape: the
+    // tuned (block size, pairs per thread) for this size class, else explicit
+    // opts, else lsm_persistent_shape.
     bool persistent = false;
-    int pgrid = 0, pmax = 0;
-    // 1024-thread blocks: the per-date barrier costs one arrival per block.
-    const int PB = 1024;
+    int pgrid = 0, pmax = 0, PB = 0;
     if (o.kernel_variant != 1) {
-        int want = 1;
-        while (want < sh.ipt && want < 8) want <<= 1;
-        for (int pm = want; pm <= 8 && !persistent; pm <<= 1) {
-            int gmax = 0;
-            if (pm == 1) gmax = max_coresident<1>(f64, src, ref, PB, ctx->cu_count);
-            else if (pm == 2) gmax = max_coresident<2>(f64, src, ref, PB, ctx->cu_count);
-            else if (pm == 4) gmax = max_coresident<4>(f64, src, ref, PB, ctx->cu_count);
-            else gmax = max_coresident<8>(f64, src, ref, PB, ctx->cu_count);
-            const int64_t g = (P.half + (int64_t)pm * PB - 1) / ((int64_t)pm * PB);
-            if (g >= 1 && g <= gmax) {
-                persistent = true;
-                pgrid = (int)g;
-                pmax = pm;
-            }
+        int bs = 0, pm = 0;
+        const bool forced = o.block_size > 0 || o.items_per_thread > 0;
+        if (forced) {
+            lsm_persistent_shape(P.half, ctx->cu_count, &bs, &pm);
+            if (o.block_size > 0) bs = o.block_size;
+            if (o.items_per_thread > 0) pm = o.items_per_thread;
+        } else {
+            pcf_tuned_shape(PCF_KIND_AMER, f64 ? 1 : 0, N, &bs, &pm);
+            if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, ctx->cu_count, &bs, &pm);
+        }
+        if (bs % 64 || bs < 64 || bs > 1024 || pm < 1 || pm > 8)
+            return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
+                            "persistent LSM shape %d x %d: need a block size in 64..1024 (multiple of 64) and "
+                            "1..8 pairs per thread", bs, pm);
+        int p2 = 1;
+        while (p2 < pm) p2 <<= 1;
+        const int64_t g = (P.half + (int64_t)p2 * bs - 1) / ((int64_t)p2 * bs);
+        int gmax;
+        if (p2 == 1) gmax = max_coresident<1>(f64, src, ref, bs, ctx->cu_count);
+        else if (p2 == 2) gmax = max_coresident<2>(f64, src, ref, bs, ctx->cu_count);
+        else if (p2 == 4) gmax = max_coresident<4>(f64, src, ref, bs, ctx->cu_count);
+        else gmax = max_coresident<8>(f64, src, ref, bs, ctx->cu_count);
+        if (g >= 1 && g <= gmax) {
+            persistent = true;
+            pgrid = (int)g;
+            pmax = p2;
+            PB = bs;
         }
         if (o.kernel_variant == 2 && !persistent)
-            return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "persistent LSM form does not fit N=%lld", (long long)N);
+            return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
+                            "persistent LSM form does not fit N=%lld (%lld blocks of %d, at most %d co-resident)",
+                            (long long)N, (long long)g, bs, gmax);
     }
     if (persistent) {
-        PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * 8 * sizeof(pcf::CSum)));
+        PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * (2 * 8 * sizeof(double) + 4 * sizeof(pcf::CSum))));
         b.partials = ctx->partials.as<pcf::CSum>();
         LsmSync sy{sync_words, sync_words + 1, sync_words + 2};
         PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, 16, ctx->stream));
