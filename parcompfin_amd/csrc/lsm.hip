@@ -108,13 +108,11 @@ __device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double*
     }
 }
 
-// Normal equations -> coef[date].  Runs in one thread.
+// Normal equations from the 8 totals g = {n, sx, sx2, sx3, sx4, sy, sxy,
+// sx2y} -> c = {flag, c0, c1, c2}.  Pure function of g: every thread that
+// calls it with the same totals gets the same coefficients.
 template <bool REF>
-__device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int date) {
-    double g[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) g[k] = tot.v[k].s + tot.v[k].c;
-    double* c = coef + 4 * date;
+__device__ __forceinline__ void lsm_solve_g(const double g[8], double c[4]) {
     const double n = g[0];
     c[1] = c[2] = c[3] = 0.0;
     if (n == 0.0) {
@@ -128,7 +126,7 @@ __device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int
     if (REF) {
 #pragma clang fp contract(off)
         // include/common.h:98-124 then :128-141, same order.
-        double x[3][3] = {{n, g[1], g[2]}, {g[1], g[2], g[3]}, {g[2], g[3], g[4]}};
+        const double x[3][3] = {{n, g[1], g[2]}, {g[1], g[2], g[3]}, {g[2], g[3], g[4]}};
         const double y[3] = {g[5], g[6], g[7]};
         double det = 0;
         for (int i = 0; i < 3; ++i)
@@ -150,25 +148,26 @@ __device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int
         c[0] = kFlagRegress;
         return;
     }
-    // Production: FP64 Cholesky of [[n,s1,s2],[s1,s2,s3],[s2,s3,s4]].
+    // Production: FP64 Cholesky of [[n,s1,s2],[s1,s2,s3],[s2,s3,s4]] with the
+    // diagonal reciprocals formed once (3 divides instead of 9).
     const double eps = 1e-13;
     const double g00 = n, g10 = g[1], g20 = g[2], g11 = g[2], g21 = g[3], g22 = g[4];
     const double b0 = g[5], b1 = g[6], b2 = g[7];
-    const double l00 = sqrt(g00);
-    const double l10 = g10 / l00, l20 = g20 / l00;
+    const double l00 = sqrt(g00), i00 = 1.0 / l00;
+    const double l10 = g10 * i00, l20 = g20 * i00;
     const double p11 = g11 - l10 * l10;
+    const double y0 = b0 * i00;
     if (p11 > eps * g11) {
-        const double l11 = sqrt(p11);
-        const double l21 = (g21 - l20 * l10) / l11;
+        const double l11 = sqrt(p11), i11 = 1.0 / l11;
+        const double l21 = (g21 - l20 * l10) * i11;
         const double p22 = g22 - l20 * l20 - l21 * l21;
+        const double y1 = (b1 - l10 * y0) * i11;
         if (p22 > eps * g22) {
-            const double l22 = sqrt(p22);
-            const double y0 = b0 / l00;
-            const double y1 = (b1 - l10 * y0) / l11;
-            const double y2 = (b2 - l20 * y0 - l21 * y1) / l22;
-            const double x2 = y2 / l22;
-            const double x1 = (y1 - l21 * x2) / l11;
-            const double x0 = (y0 - l10 * x1 - l20 * x2) / l00;
+            const double i22 = 1.0 / sqrt(p22);
+            const double y2 = (b2 - l20 * y0 - l21 * y1) * i22;
+            const double x2 = y2 * i22;
+            const double x1 = (y1 - l21 * x2) * i11;
+            const double x0 = (y0 - l10 * x1 - l20 * x2) * i00;
             c[1] = x0;
             c[2] = x1;
             c[3] = x2;
@@ -176,16 +175,25 @@ __device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int
             return;
         }
         // {1, x}
-        const double y0 = b0 / l00;
-        const double y1 = (b1 - l10 * y0) / l11;
-        const double x1 = y1 / l11;
-        c[1] = (y0 - l10 * x1) / l00;
+        const double x1 = y1 * i11;
+        c[1] = (y0 - l10 * x1) * i00;
         c[2] = x1;
         c[0] = kFlagReduced;
         return;
     }
     c[1] = b0 / n;  // {1}: the mean continuation value
     c[0] = kFlagReduced;
+}
+
+// Normal equations -> coef[date] in global memory.  Runs in one thread.
+template <bool REF>
+__device__ void lsm_solve(const pcf::Acc<8>& tot, double* __restrict__ coef, int date) {
+    double g[8], c[4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = tot.v[k].s + tot.v[k].c;
+    lsm_solve_g<REF>(g, c);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) coef[4 * date + k] = c[k];
 }
 
 // Exercise decision at date m for one path.  Returns true if state changed.
@@ -538,61 +546,114 @@ __device__ __forceinline__ void block_sum_plain(double v[K], double* lds) {
     }
 }
 
-// Per-date all-reduce of the 8 regression sums.  On entry thread 0 holds the
-// block's sums in v; on return thread 0 holds the grid totals, identical in
-// every block.  `slots` holds one row of 8 per block for this date's parity;
-// `target` = arrivals that complete this date.  The partials are published
-// with coherent stores that complete before the arrival is counted, and read
-// with coherent loads after the count is seen complete.
-template <int K>
-__device__ __forceinline__ void lsm_allreduce(double v[K], double* __restrict__ slots, const LsmSync& s,
-                                              unsigned int target, double* ldsp) {
-    if (threadIdx.x == 0) {
+// Sum of 8 values over the 64 lanes of a wave by transposition: at each of
+// the xor-32/16/8 steps a lane keeps half of its remaining sums and adds its
+// partner's copy of them; xor-4/2/1 then finish a plain tree.  Afterwards
+// every lane whose bits 3..5 encode k (k = b3 + 2 b4 + 4 b5) holds sum k.
+// 10 FP64 exchanges instead of 48; IEEE addition commutes, so both partners
+// of an exchange form the same sum and the order is fixed.
+__device__ __forceinline__ double wave_sum8(const double v[8]) {
+    const int lane = threadIdx.x & (pcf::kWave - 1);
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    double a[4];
 #pragma unroll
-        for (int k = 0; k < K; ++k) pcf::publish(slots + (size_t)blockIdx.x * K + k, v[k]);
+    for (int j = 0; j < 4; ++j) {
+        const double keep = b5 ? v[j + 4] : v[j];
+        const double give = b5 ? v[j] : v[j + 4];
+        a[j] = keep + __shfl_xor(give, 32, pcf::kWave);
+    }
+    double c[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double keep = b4 ? a[j + 2] : a[j];
+        const double give = b4 ? a[j] : a[j + 2];
+        c[j] = keep + __shfl_xor(give, 16, pcf::kWave);
+    }
+    const double keep = b3 ? c[1] : c[0];
+    const double give = b3 ? c[0] : c[1];
+    double d = keep + __shfl_xor(give, 8, pcf::kWave);
+    d += __shfl_xor(d, 4, pcf::kWave);
+    d += __shfl_xor(d, 2, pcf::kWave);
+    d += __shfl_xor(d, 1, pcf::kWave);
+    return d;
+}
+
+// Block sum of 8 values: the wave sums above, then the waves in order, one
+// lane per sum.  Returns sum k in thread k < 8 (0 elsewhere).  `lds` holds
+// 16 waves x 8 doubles.
+__device__ __forceinline__ double block_sum8(const double v[8], double* lds) {
+    const double d = wave_sum8(v);
+    const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
+    if ((lane & 7) == 0) lds[wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < 8) {
+        const int nw = (blockDim.x + pcf::kWave - 1) / pcf::kWave;
+        t = lds[threadIdx.x];
+        for (int w = 1; w < nw; ++w) t += lds[w * 8 + threadIdx.x];
+    }
+    return t;
+}
+
+// Per-date all-reduce of the 8 regression sums acc (per thread) into g (the
+// grid totals, in every thread of every block, bit-identical everywhere).
+// `slots` holds this date's parity, sum-major: slots[k * grid + block], so
+// the 8 publishing lanes write one value each and a wave's load of sum k
+// over consecutive blocks is one contiguous run; `target` = arrivals that
+// complete this date.  The partials are published with coherent stores that
+// complete before the arrival is counted, and read with coherent loads after
+// the count is seen complete.
+__device__ __forceinline__ void lsm_allreduce(const double acc[8], double g[8], double* __restrict__ slots,
+                                              const LsmSync& s, unsigned int target, double* red_a,
+                                              double* red_b, double* tot_sh) {
+    const double mine = block_sum8(acc, red_a);
+    if (threadIdx.x < 8) {
+        pcf::publish(slots + (size_t)threadIdx.x * gridDim.x + blockIdx.x, mine);
         pcf::stores_complete();
-        __hip_atomic_fetch_add(s.arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned int it = 0;
-        while (pcf::fetch_u32(s.arrive) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            // Past the bound, or once any block has given up, stop waiting.
-            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
-                atomicOr(s.err, 1u);
-                break;
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_add(s.arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned int it = 0;
+            while (pcf::fetch_u32(s.arrive) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                // Past the bound, or once any block has given up, stop waiting.
+                if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
+                    atomicOr(s.err, 1u);
+                    break;
+                }
             }
         }
     }
     __syncthreads();
-    double t[K];
+    double t[8];
 #pragma unroll
-    for (int k = 0; k < K; ++k) t[k] = 0.0;
+    for (int k = 0; k < 8; ++k) t[k] = 0.0;
     for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < K; ++k) t[k] += pcf::fetch(slots + (size_t)bb * K + k);
-    block_sum_plain<K>(t, ldsp);
+        for (int k = 0; k < 8; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
+    const double tot = block_sum8(t, red_b);
+    if (threadIdx.x < 8) tot_sh[threadIdx.x] = tot;
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = t[k];
+    for (int k = 0; k < 8; ++k) g[k] = tot_sh[k];
 }
 
-// Discount factors exp(-r dt k) served from LDS when M + 1 fits.
+// Discount factors exp(-r dt k), k = 0..M, live in LDS (the persistent form
+// needs M + 1 <= kDiscLds; longer schedules run as the graph of launches).
 constexpr int kDiscLds = 1024;
 
 template <typename Real, int SRC, bool REF, int PMAX>
 __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) {
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
-    __shared__ double cm_sh[4];
-    __shared__ double disc_sh[kDiscLds];
+    __shared__ double red_a[16 * 8], red_b[16 * 8], tot_sh[8];
+    __shared__ double disc[kDiscLds];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool disc_in_lds = M + 1 <= kDiscLds;
-    if (disc_in_lds)
-        for (int k = threadIdx.x; k <= M; k += blockDim.x) disc_sh[k] = b.disc[k];
-    const double* __restrict__ disc = disc_in_lds ? disc_sh : b.disc;
+    for (int k = threadIdx.x; k <= M; k += blockDim.x) disc[k] = b.disc[k];
     // Date slots: 2 parities x grid x 8 doubles, then the final sums' slots.
     double* __restrict__ slots = (double*)b.partials;
     __syncthreads();
@@ -683,19 +744,12 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
         }
         if constexpr (kPrefetch)
             if (m > 1) load_row(m - 1, Sn);
-        block_sum_plain<8>(acc, ldsp);
-        lsm_allreduce<8>(acc, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen * gridDim.x, ldsp);
-        if (threadIdx.x == 0) {
-            pcf::Acc<8> tot;
+        double g[8], cm[4];
+        lsm_allreduce(acc, g, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen * gridDim.x, red_a, red_b, tot_sh);
+        lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
+        if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
-            for (int k = 0; k < 8; ++k) tot.v[k] = {acc[k], 0.0};
-            lsm_solve<REF>(tot, cm_sh, 0);
-            if (blockIdx.x == 0)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm_sh[k];
-        }
-        __syncthreads();
-        const double cm[4] = {cm_sh[0], cm_sh[1], cm_sh[2], cm_sh[3]};
+            for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
         if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
 #pragma unroll
             for (int k = 0; k < PMAX; ++k) {
@@ -714,7 +768,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
         if constexpr (kPrefetch)
 #pragma unroll
             for (int i = 0; i < 2 * PMAX; ++i) Sc[i] = Sn[i];
-        __syncthreads();  // cm_sh is rewritten at the next date
+        // (tot_sh and the reduction buffers are rewritten only after the
+        // barriers of the next date, which every thread reaches after this
+        // date's reads)
     }
     // Final: discounted booked cash flows + European control.
     double s = 0.0, s2 = 0.0;
@@ -1032,7 +1088,10 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     // opts, else lsm_persistent_shape.
     bool persistent = false;
     int pgrid = 0, pmax = 0, PB = 0;
-    if (o.kernel_variant != 1) {
+    if (o.kernel_variant == 2 && M + 1 > kDiscLds)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "persistent LSM form holds at most %d dates (M=%d)", kDiscLds - 1,
+                        M);
+    if (o.kernel_variant != 1 && M + 1 <= kDiscLds) {
         int bs = 0, pm = 0;
         const bool forced = o.block_size > 0 || o.items_per_thread > 0;
         if (forced) {
