@@ -19,9 +19,15 @@ p50/p99 latency overall and for 2^16-path requests.
 Every configuration is also priced at its full size in FP32/Kahan and FP64
 with the error in standard errors and the closed-form checks.
 
-roofline: the dominant kernel of the served mix by device time, measured
-at its full configuration with HIP events recorded by the library on the
-stream it launches on.
+roofline: the dominant kernel of the served mix by device time (the kind
+with the largest summed kernel time in the FP32/Kahan steps), measured at its
+full configuration with HIP events recorded by the library on the stream it
+launches on.  `achieved` is the algorithmic count per launch (bytes for the
+American path store, flops otherwise) / the mean launch time; `traffic` is
+the HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of the
+same kernel, each scaled by what the counter reads for a copy of known size
+(profiles/pmc_traffic.json, written from those runs).  Every kernel's
+roofline object is also listed under "rooflines".
 
 cpu_baseline: the reference's own OpenMP programs (src/mc_eur_omp.cpp,
 src/mc_asia_omp.cpp, src/mc_amer_omp.cpp built from source at -O2 -fopenmp
@@ -193,6 +199,19 @@ def per_config(eng):
     return out
 
 
+def load_traffic():
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def traffic_of(kernel):
+    t = load_traffic().get(kernel)
+    return t["bytes_per_launch"] if t else None
+
+
 def asia_roofline(eng, K=10):
     """k_asia<fp32k> at 2^22 x 252: algorithmic FP32 flops / HIP-event time."""
     N, M = 1 << 22, 252
@@ -203,9 +222,40 @@ def asia_roofline(eng, K=10):
     return {"bound": "mfma",
             "pipe": "VALU FP32 (non-packed FMA + transcendental units); 157.3 TFLOP/s is MI355X's FP32 peak",
             "kernel": "k_asia<fp32k> (mc_asia, 2^22 paths x 252 steps)", "achieved": ach,
-            "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": None,
+            "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": traffic_of("k_asia<0>"),
             "kernel_ms": kms, "flops_per_launch": fl,
             "flops_per_unit": "14 per path-step (normal 6, I update 4, S update 4)"}
+
+
+def lsm_roofline(eng, K=10):
+    """k_lsm_persistent<float> at 2^20 paths x 50 dates: algorithmic bytes / HIP-event time."""
+    N, M = 1 << 20, 50
+    rs = [eng.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=200 + k) for k in range(K + 1)][1:]
+    kms = sum(r.kernel_ms for r in rs) / K
+    by = rs[0].bytes
+    ach = by / (kms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": f"k_lsm_persistent<float> (mc_amer put, 2^20 paths x 50 dates, grid "
+                                      f"{rs[0].grid_blocks}x{rs[0].block_size})",
+            "achieved": ach, "peak": PEAK["hbm"], "unit": "GB/s", "frac": ach / PEAK["hbm"],
+            "traffic": traffic_of("k_lsm_persistent<float>"), "kernel_ms": kms, "bytes_per_launch": by,
+            "bytes_per_unit": "24 per path-date in FP32: store write 4 + backward read 4 + per-path state "
+                              "{cash flow 4, date 4} read and written (the state is register-resident here, "
+                              "so traffic is below this count)"}
+
+
+def basket_roofline(eng, K=10):
+    """k_basket_mfma at d=32, 2^24 paths: algorithmic FP32 flops / HIP-event time."""
+    import numpy as np
+    sig = np.random.default_rng(32).uniform(0.15, 0.35, size=32)
+    rs = [eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", 32, 0.3, sigmas=sig, seed=300 + k)
+          for k in range(K + 1)][1:]
+    kms = sum(r.kernel_ms for r in rs) / K
+    fl = rs[0].flops
+    ach = fl / (kms * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": "k_basket_mfma (mc_eur_multi d=32, 2^24 paths)", "achieved": ach,
+            "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
+            "traffic": traffic_of("k_basket_mfma"), "kernel_ms": kms, "flops_per_launch": fl,
+            "flops_per_unit": "d^2 + 12 d + 3 per path (normals 6d, Z L^T d(d+1), asset 5d, payoff 3)"}
 
 
 def profile_one(eng, which, K, W):
@@ -315,8 +365,9 @@ def main():
         log(f"autotune: {len(tuned)} entries in {time.perf_counter() - t0:.1f} s")
 
     configs = {} if args.no_configs else per_config(eng)
-    roof = asia_roofline(eng)
-    log("roofline:", roof)
+    rooflines = {"asia": asia_roofline(eng), "amer": lsm_roofline(eng), "basket": basket_roofline(eng)}
+    for k, v in rooflines.items():
+        log(f"roofline {k}:", v)
 
     mix = make_mix(2026, args.requests)
     ps = path_steps(mix)
@@ -342,6 +393,10 @@ def main():
         k["requests"] += 1
         k["paths_steps"] += q["args"][5] * q["steps"]
         k["device_ms"] += a.kernel_ms / max(a.batch_size, 1)
+
+    dominant = max(by_kind, key=lambda k: by_kind[k]["device_ms"])
+    roof = dict(rooflines.get(dominant, rooflines["amer"]), dominant_kind=dominant)
+    log("roofline (dominant kernel of the mix):", roof)
 
     # Poisson latency phase at ~75% of capacity.
     req_rate = len(mix) * K / sum(t32)
@@ -390,6 +445,7 @@ def main():
                    "requests_per_step": len(mix), "paths_steps_per_step": ps, "slots": args.slots,
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": 1e3 * sum(t64) / K, "fp32k_over_fp64": value / value64},
         "accuracy": {"max_abs_fp32k_minus_fp64_in_se": max(dse) if dse else None,

@@ -193,7 +193,9 @@ typedef struct pcf_tune_entry {
 
 /* Sweeps block size and items per thread for the Philox FP32/Kahan forms of
  * the kinds in kind_mask (bit k = enum pcf_kind k; EUR, ASIA, BASKET, AMER) at
- * size classes 2^14 .. 2^max_log2_paths, times each candidate on the device,
+ * size classes 2^14 .. 2^max_log2_paths (AMER: block size and antithetic
+ * pairs per thread of the persistent form, at the sizes it holds), times each
+ * candidate on the device,
  * and installs the fastest shape in the process-wide table every context
  * (and the server) consults when opts leave block_size / items_per_thread 0.
  * A shape change moves the reduction tree, so results change in the last

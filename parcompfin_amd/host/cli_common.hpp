@@ -6,8 +6,8 @@
 // and prints the reference's one CSV row (include/common.h:212-249):
 //     Method,Payoff,S0,E,r,sigma,T,N,M,Parallel,Nr_of_assets,T_overall,
 //     T_calculation,Result,Abs_Error,Error
-// with 10 significant digits.  Method is HIP (FP32/Kahan) or HIP_fp64, and
-// Parallel is the number of GPUs (1).  Everything else is an optional
+// with 10 significant digits.  Method is HIP (FP32/Kahan), HIP_fp64 or
+// HIP_verify (--rng mt19937), and Parallel is the number of GPUs (1).  Everything else is an optional
 // trailing flag, so runscript-style invocations keep working:
 //     --seed S          Philox key / mt19937 seed (default: time(), as the reference)
 //     --precision P     fp32k (default) | fp64
