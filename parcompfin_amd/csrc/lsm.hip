@@ -481,8 +481,17 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
 // Every wait is bounded: past the limit a block sets the error word and stops
 // waiting (the host reports PCF_ERR_TIMEOUT), so every wave always reaches
 // the end of the kernel.
+// Per-date arrivals are spread over kArrivalCounters counters, each on its
+// own 128-byte line (block b counts on counter b % kArrivalCounters): device-
+// scope atomics on one address serialise at the memory side (~25 ns each,
+// measured), so one shared counter costs ~grid x 25 ns per date.
+constexpr int kArrivalCounters = 32;
+constexpr int kCounterStride = 32;  // u32 words = 128 bytes
+constexpr int kSyncWords = kArrivalCounters * kCounterStride + 32;  // + fin, err
+
 struct LsmSync {
-    unsigned int* arrive;  // per-date arrivals, growing by grid each date
+    unsigned int* arrive;  // kArrivalCounters counters, kCounterStride apart; each grows
+                           // by its block count per date
     unsigned int* fin;     // arrivals at the final sums
     unsigned int* err;     // nonzero once any wait gave up
 };
@@ -599,27 +608,36 @@ __device__ __forceinline__ double block_sum8(const double v[8], double* lds) {
 // grid totals, in every thread of every block, bit-identical everywhere).
 // `slots` holds this date's parity, sum-major: slots[k * grid + block], so
 // the 8 publishing lanes write one value each and a wave's load of sum k
-// over consecutive blocks is one contiguous run; `target` = arrivals that
-// complete this date.  The partials are published with coherent stores that
+// over consecutive blocks is one contiguous run; `gen` = dates completed
+// including this one.  The partials are published with coherent stores that
 // complete before the arrival is counted, and read with coherent loads after
-// the count is seen complete.
+// every counter is seen complete (wave 0 polls all counters at once).
 __device__ __forceinline__ void lsm_allreduce(const double acc[8], double g[8], double* __restrict__ slots,
-                                              const LsmSync& s, unsigned int target, double* red_a,
+                                              const LsmSync& s, unsigned int gen, double* red_a,
                                               double* red_b, double* tot_sh) {
     const double mine = block_sum8(acc, red_a);
-    if (threadIdx.x < 8) {
-        pcf::publish(slots + (size_t)threadIdx.x * gridDim.x + blockIdx.x, mine);
+    if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
+        const int lane = threadIdx.x;
+        if (lane < 8) pcf::publish(slots + (size_t)lane * gridDim.x + blockIdx.x, mine);
         pcf::stores_complete();
-        if (threadIdx.x == 0) {
-            __hip_atomic_fetch_add(s.arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned int it = 0;
-            while (pcf::fetch_u32(s.arrive) < target) {
-                __builtin_amdgcn_s_sleep(1);
-                // Past the bound, or once any block has given up, stop waiting.
-                if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
-                    atomicOr(s.err, 1u);
-                    break;
-                }
+        if (lane == 0)
+            __hip_atomic_fetch_add(s.arrive + (blockIdx.x % kArrivalCounters) * kCounterStride, 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Lane c < kArrivalCounters waits for counter c to reach (its blocks) x gen.
+        const unsigned int blocks_c =
+            lane < kArrivalCounters && (unsigned)lane < gridDim.x
+                ? (gridDim.x - 1 - (unsigned)lane) / kArrivalCounters + 1 : 0u;
+        const unsigned int target = blocks_c * gen;
+        unsigned int it = 0;
+        while (true) {
+            const unsigned int v =
+                lane < kArrivalCounters ? pcf::fetch_u32(s.arrive + lane * kCounterStride) : 0u;
+            if (__all(v >= target)) break;
+            __builtin_amdgcn_s_sleep(1);
+            // Past the bound, or once any block has given up, stop waiting.
+            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
+                if (lane == 0) atomicOr(s.err, 1u);
+                break;
             }
         }
     }
@@ -745,7 +763,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
         if constexpr (kPrefetch)
             if (m > 1) load_row(m - 1, Sn);
         double g[8], cm[4];
-        lsm_allreduce(acc, g, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen * gridDim.x, red_a, red_b, tot_sh);
+        lsm_allreduce(acc, g, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen, red_a, red_b, tot_sh);
         lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
         if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
@@ -979,9 +997,10 @@ int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exa
     pcf_opts_init(&o);
     const LsmShape sh = lsm_shape(N, f64 ? 1 : 0, o);
     const int64_t grid_max = std::max(sh.grid_paths, sh.grid_step);
-    // + the persistent form's partials: at most 4096 co-resident blocks x 192 B
+    // + parameters and sync words (< 8 KB) + the persistent form's partials:
+    // at most 4096 co-resident blocks x 192 B
     return (int64_t)M * sh.cap * rs + sh.cap * (4 + 2 * rs) + grid_max * 10 * 16 +
-           8 * (5 * (int64_t)(M + 1)) + 4096 + 4096 * 192;
+           8 * (5 * (int64_t)(M + 1)) + 8192 + 4096 * 192;
 }
 
 // Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
@@ -1045,7 +1064,9 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
     // coef (4 per date) | discount table | parameter block | sync words
-    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) + 64;
+    // coef | disc | params | (256-byte aligned) persistent-form sync words
+    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) + 512 +
+                              sizeof(unsigned int) * kSyncWords;
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
@@ -1081,7 +1102,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
                                       hipMemcpyHostToDevice, ctx->stream));
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
 
-    unsigned int* sync_words = (unsigned int*)((char*)Pdev + ((sizeof(LsmParams) + 15) & ~(size_t)15));
+    unsigned int* sync_words =
+        (unsigned int*)(((uintptr_t)((char*)Pdev + sizeof(LsmParams)) + 255) & ~(uintptr_t)255);
 
     // Persistent form when the request fits a co-resident grid.  Shape: the
     // tuned (block size, pairs per thread) for this size class, else explicit
@@ -1128,8 +1150,9 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     if (persistent) {
         PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * (2 * 8 * sizeof(double) + 4 * sizeof(pcf::CSum))));
         b.partials = ctx->partials.as<pcf::CSum>();
-        LsmSync sy{sync_words, sync_words + 1, sync_words + 2};
-        PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, 16, ctx->stream));
+        const int fin_w = kArrivalCounters * kCounterStride;
+        LsmSync sy{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
+        PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, sizeof(unsigned int) * kSyncWords, ctx->stream));
         {
             std::lock_guard<std::mutex> pl(persistent_mu());
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
@@ -1202,11 +1225,12 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     if (o.when_out)
         PCF_HIP_CHECK(ctx, hipMemcpyAsync(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost,
                                           ctx->stream));
-    unsigned int sync_host[4] = {0, 0, 0, 0};
+    unsigned int sync_host[4] = {0, 0, 0, 0};  // fin, err
     if (persistent)
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_host, sync_words, sizeof sync_host, hipMemcpyDeviceToHost, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_host, sync_words + kArrivalCounters * kCounterStride, sizeof sync_host,
+                                          hipMemcpyDeviceToHost, ctx->stream));
     PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (sync_host[2])
+    if (sync_host[1])
         return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
     int small = 0, fallback = 0, singular = 0;
     for (int m = 1; m <= M - 1; ++m) {
