@@ -74,21 +74,45 @@ __device__ __forceinline__ Real pay(Real S, Real E, Real pf) {
     return pcf::payoff<Real>(S, E, pf);
 }
 
-// Regression sums of one path at date m into acc[0..7].
+// Regression sums of one path at date m, in two halves: the x-moments
+// acc[0..4] = {n, x, x^2, x^3, x^4} depend only on the path value S at date m;
+// the y-terms acc[5..7] = {y, yx, yx^2} also on the exercise state after date
+// m+1.  Each sum receives the same terms in the same path order whichever
+// half runs first, so the persistent kernel can form the x-moments of the
+// next date while it waits for the other blocks.
 template <typename Real, bool REF>
-__device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double* __restrict__ disc,
-                                               int m, Real S, int32_t when, Real cf, double acc[8]) {
+__device__ __forceinline__ void lsm_accumulate_x(const LsmParams& P, Real S, double acc[5]) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    if (pv > 0.0) {
+        if (REF) {
+#pragma clang fp contract(off)
+            const double x = (double)S - P.E;
+            acc[0] += 1.0;
+            acc[1] += x;
+            acc[2] += x * x;
+            acc[3] += x * x * x;
+            acc[4] += x * x * x * x;
+        } else {
+            const double x = (double)S * P.invE;
+            const double x2 = x * x;
+            acc[0] += 1.0;
+            acc[1] += x;
+            acc[2] += x2;
+            acc[3] += x2 * x;
+            acc[4] += x2 * x2;
+        }
+    }
+}
+
+template <typename Real, bool REF>
+__device__ __forceinline__ void lsm_accumulate_y(const LsmParams& P, const double* __restrict__ disc, int m,
+                                                 Real S, int32_t when, Real cf, double acc[8]) {
     const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
     if (pv > 0.0) {
         if (REF) {
 #pragma clang fp contract(off)
             const double x = (double)S - P.E;
             const double y = exp(P.nrdt * ((double)when - (double)m)) * (double)cf;
-            acc[0] += 1.0;
-            acc[1] += x;
-            acc[2] += x * x;
-            acc[3] += x * x * x;
-            acc[4] += x * x * x * x;
             acc[5] += y;
             acc[6] += y * x;
             acc[7] += y * x * x;
@@ -96,16 +120,18 @@ __device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double*
             const double x = (double)S * P.invE;
             const double y = disc[when - m] * (double)cf;
             const double x2 = x * x;
-            acc[0] += 1.0;
-            acc[1] += x;
-            acc[2] += x2;
-            acc[3] += x2 * x;
-            acc[4] += x2 * x2;
             acc[5] += y;
             acc[6] += y * x;
             acc[7] += y * x2;
         }
     }
+}
+
+template <typename Real, bool REF>
+__device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double* __restrict__ disc,
+                                               int m, Real S, int32_t when, Real cf, double acc[8]) {
+    lsm_accumulate_x<Real, REF>(P, S, acc);
+    lsm_accumulate_y<Real, REF>(P, disc, m, S, when, cf, acc);
 }
 
 // Normal equations from the 8 totals g = {n, sx, sx2, sx3, sx4, sy, sxy,
@@ -604,18 +630,18 @@ __device__ __forceinline__ double block_sum8(const double v[8], double* lds) {
     return t;
 }
 
-// Per-date all-reduce of the 8 regression sums acc (per thread) into g (the
-// grid totals, in every thread of every block, bit-identical everywhere).
-// `slots` holds this date's parity, sum-major: slots[k * grid + block], so
-// the 8 publishing lanes write one value each and a wave's load of sum k
-// over consecutive blocks is one contiguous run; `gen` = dates completed
-// including this one.  The partials are published with coherent stores that
-// complete before the arrival is counted, and read with coherent loads after
-// every counter is seen complete (wave 0 polls all counters at once).
-__device__ __forceinline__ void lsm_allreduce(const double acc[8], double g[8], double* __restrict__ slots,
-                                              const LsmSync& s, unsigned int gen, double* red_a,
-                                              double* red_b, double* tot_sh) {
-    const double mine = block_sum8(acc, red_a);
+// Per-date all-reduce of the 8 regression sums, in three steps so that the
+// kernel can work between the arrival and the wait:
+//   lsm_publish: the block's 8 sums (in threads 0..7) go to this date's slots,
+//     sum-major (slots[k * grid + block], so a wave's load of sum k over
+//     consecutive blocks is one contiguous run), with coherent stores that
+//     complete before the block's arrival is counted;
+//   lsm_wait_all: wave 0 polls every arrival counter until each has counted
+//     its blocks `gen` times (dates completed including this one);
+//   lsm_gather: every block reads all partials with coherent loads and sums
+//     them in the same fixed order, so g (the grid totals, in every thread)
+//     is bit-identical in every block.
+__device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ slots, const LsmSync& s) {
     if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
         const int lane = threadIdx.x;
         if (lane < 8) pcf::publish(slots + (size_t)lane * gridDim.x + blockIdx.x, mine);
@@ -623,6 +649,12 @@ __device__ __forceinline__ void lsm_allreduce(const double acc[8], double g[8], 
         if (lane == 0)
             __hip_atomic_fetch_add(s.arrive + (blockIdx.x % kArrivalCounters) * kCounterStride, 1u,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ void lsm_wait_all(const LsmSync& s, unsigned int gen) {
+    if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
+        const int lane = threadIdx.x;
         // Lane c < kArrivalCounters waits for counter c to reach (its blocks) x gen.
         const unsigned int blocks_c =
             lane < kArrivalCounters && (unsigned)lane < gridDim.x
@@ -642,6 +674,10 @@ __device__ __forceinline__ void lsm_allreduce(const double acc[8], double g[8], 
         }
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict__ slots, double* red_b,
+                                           double* tot_sh) {
     double t[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) t[k] = 0.0;
@@ -731,10 +767,11 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             stb[2 * k + 1] = cf[2 * k + 1];
         }
     }
-    // Backward sweep: one barrier-reduce per exercise date.  Sc holds the
-    // owned paths' values at date m; the next date's row is loaded into Sn
-    // before the barrier so its latency hides behind the wait.
-    // (no prefetch at 8 pairs per thread: its registers would spill)
+    // Backward sweep: one grid all-reduce per exercise date.  Sc holds the
+    // owned paths' values at date m.  While a block waits for the others at
+    // date m it loads the row of date m-1 (into Sn; at 8 pairs per thread the
+    // registers would spill, so the row is read twice there) and forms that
+    // date's x-moments xm, which do not depend on the decisions at m.
     constexpr bool kPrefetch = PMAX <= 4;
     Real Sc[2 * PMAX], Sn[kPrefetch ? 2 * PMAX : 1];
     auto load_row = [&](int m, Real* dst) {
@@ -746,24 +783,50 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             dst[2 * k + 1] = own ? store[(int64_t)(m - 1) * N + n + half] : Real(0);
         }
     };
-    if (kPrefetch && M >= 2) load_row(M - 1, Sc);
+    auto x_moments = [&](const Real* row, double xm[5]) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) xm[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k)
+            if (t0 + (int64_t)k * T < half) {
+                lsm_accumulate_x<Real, REF>(P, row[2 * k], xm);
+                lsm_accumulate_x<Real, REF>(P, row[2 * k + 1], xm);
+            }
+    };
+    double xm[5] = {0, 0, 0, 0, 0};
+    if (M >= 2) {
+        load_row(M - 1, Sc);
+        x_moments(Sc, xm);
+    }
     unsigned int gen = 0;
     for (int m = M - 1; m >= 1; --m) {
         ++gen;
-        if constexpr (!kPrefetch) load_row(m, Sc);
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if constexpr (!kPrefetch)
+            if (m != M - 1) load_row(m, Sc);
+        double acc[8] = {xm[0], xm[1], xm[2], xm[3], xm[4], 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
             const int64_t n = t0 + (int64_t)k * T;
             if (n < half) {
-                lsm_accumulate<Real, REF>(P, disc, m, Sc[2 * k], when[2 * k], cf[2 * k], acc);
-                lsm_accumulate<Real, REF>(P, disc, m, Sc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
+                lsm_accumulate_y<Real, REF>(P, disc, m, Sc[2 * k], when[2 * k], cf[2 * k], acc);
+                lsm_accumulate_y<Real, REF>(P, disc, m, Sc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
             }
         }
-        if constexpr (kPrefetch)
-            if (m > 1) load_row(m - 1, Sn);
+        double* const slots_m = slots + (size_t)(m & 1) * gridDim.x * 8;
+        lsm_publish(block_sum8(acc, red_a), slots_m, sy);
+        if (m > 1) {
+            if constexpr (kPrefetch) {
+                load_row(m - 1, Sn);
+                x_moments(Sn, xm);
+            } else {
+                Real Sx[2 * PMAX];
+                load_row(m - 1, Sx);
+                x_moments(Sx, xm);
+            }
+        }
+        lsm_wait_all(sy, gen);
         double g[8], cm[4];
-        lsm_allreduce(acc, g, slots + (size_t)(m & 1) * gridDim.x * 8, sy, gen, red_a, red_b, tot_sh);
+        lsm_gather(g, slots_m, red_b, tot_sh);
         lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
         if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
@@ -906,18 +969,16 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o) {
 
 // Default shape of the persistent form.  Per date every block reads every
 // block's partials (grid^2 coherent loads) and each CU works through its own
-// pairs, so the grid targets 64 blocks for up to 2^13 pairs, 128 up to 2^17
-// and one per CU beyond; pairs per block are then split into at most 4 per
-// thread (8 only when a 1024-thread block is not enough).
+// pairs with latency exposed at few waves, so the grid targets 64 blocks for
+// up to 2^13 antithetic pairs, 128 up to 2^15 and one per CU beyond, and the
+// pairs of a block are spread over as many threads as a block holds (1024)
+// before a thread takes more than one pair (1, 2, 4, 8).
 void lsm_persistent_shape(int64_t half, int cus, int* bs, int* pm) {
-    const int64_t target = half <= (1 << 13) ? 64 : half <= (1 << 17) ? 128 : std::max(cus, 64);
+    const int64_t target = half <= (1 << 13) ? 64 : half <= (1 << 15) ? 128 : std::max(cus, 64);
     const int64_t per_block = (half + target - 1) / target;
-    int p = (int)std::min<int64_t>(4, std::max<int64_t>(1, per_block / 256));
-    int64_t b = ((per_block + p - 1) / p + 63) / 64 * 64;
-    if (b > 1024) {
-        p = 8;
-        b = ((per_block + 7) / 8 + 63) / 64 * 64;
-    }
+    int p = 1;
+    while (p < 8 && (per_block + p - 1) / p > 1024) p <<= 1;
+    const int64_t b = ((per_block + p - 1) / p + 63) / 64 * 64;
     *pm = p;
     *bs = (int)std::min<int64_t>(1024, std::max<int64_t>(64, b));
 }
