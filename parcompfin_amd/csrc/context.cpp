@@ -59,11 +59,13 @@ int pcf_ctx_create(int device, pcf_ctx** out) {
     ctx->cu_count = prop.multiProcessorCount;
     ctx->total_mem = (int64_t)prop.totalGlobalMem;
     std::snprintf(ctx->name, sizeof ctx->name, "%s (%s)", prop.name, prop.gcnArchName);
-    // Default-priority stream (no priority games: see the design notes).
+    // Non-blocking stream at the default priority (every stream here keeps it).
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(e, "hipStreamCreate");
     for (hipEvent_t* ev : {&ctx->ev_begin, &ctx->ev_kbeg, &ctx->ev_kend, &ctx->ev_end})
         if ((e = hipEventCreate(ev)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreateWithFlags(&ctx->ev_ready, hipEventDisableTiming)) != hipSuccess)
+        return bail(e, "hipEventCreate");
     if ((e = ctx->counters.ensure(kCounterSlots * sizeof(unsigned int), true)) != hipSuccess)
         return bail(e, "alloc counters");
     if ((e = ctx->out.ensure(kOutDoubles * sizeof(double), true)) != hipSuccess)
@@ -88,7 +90,7 @@ int pcf_ctx_destroy(pcf_ctx* ctx) {
         b->release();
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     if (ctx->h_jobs) (void)hipHostFree(ctx->h_jobs);
-    for (hipEvent_t ev : {ctx->ev_begin, ctx->ev_kbeg, ctx->ev_kend, ctx->ev_end})
+    for (hipEvent_t ev : {ctx->ev_begin, ctx->ev_kbeg, ctx->ev_kend, ctx->ev_end, ctx->ev_ready})
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

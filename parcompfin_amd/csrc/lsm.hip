@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
 }
 
 // --- persistent form ----------------------------------------------------------
-// One cooperative launch prices the whole request.  Thread t owns antithetic
+// One launch prices the whole request (grid co-resident).  Thread t owns antithetic
 // pairs n = t, t + T, ... (T threads in the grid, at most PMAX pairs each) for
 // the forward path generation and the whole backward sweep, so the per-path
 // state (exercise date, true and booked cash flows) stays in registers and
@@ -892,8 +893,13 @@ hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, in
     LsmBufs bb = b;
     LsmSync ss = sy;
     void* args[] = {&bb, &ss};
-    return hipLaunchCooperativeKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
-                                      dim3((unsigned)B), args, 0, s);
+    // A plain launch: the grid fits the device's co-resident capacity (checked
+    // by the caller), persistent launches run one after another on one stream
+    // per device, and every other kernel of the library finishes on its own,
+    // so all blocks become resident -- at worst after a co-running kernel
+    // ends.  Every wait is bounded all the same.
+    return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
+                           dim3((unsigned)B), args, 0, s);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX>
@@ -927,8 +933,27 @@ int max_coresident(bool f64, int src, bool ref, int B, int cus) {
                : max_coresident_t<double, kPhilox, false, PMAX>(B, cus);
 }
 
-// One persistent LSM kernel on the device at a time (two of them could each
-// hold part of the CUs while waiting for their remaining blocks).
+// Persistent LSM kernels run one at a time per device (two of them could each
+// hold part of the CUs while waiting for their remaining blocks): every
+// context launches them on one process-wide stream per device, in order,
+// handing over with events, so no host thread waits on another's kernel.
+hipError_t coop_stream(int device, hipStream_t* out) {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = streams.find(device);
+    if (it != streams.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    hipStream_t st = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);  // default priority
+    if (e != hipSuccess) return e;
+    streams[device] = st;
+    *out = st;
+    return hipSuccess;
+}
+// Serialises enqueueing onto that stream (launch + its bracketing events).
 std::mutex& persistent_mu() {
     static std::mutex m;
     return m;
@@ -1159,9 +1184,19 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     if ((st = pcf_timing_begin(ctx))) return st;
     if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
     b.hw = src == kHost ? ctx->normals.as<double>() : nullptr;
-    PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, host_blk.data(), host_blk.size() * sizeof(double),
-                                      hipMemcpyHostToDevice, ctx->stream));
-    PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+    // coef | disc | params are contiguous on the device: one copy, from the
+    // context's pinned staging buffer when it fits (the call synchronises
+    // before returning, so the staging is free again for the next call).
+    const size_t blk_bytes = host_blk.size() * sizeof(double);
+    if (blk_bytes + sizeof P <= kMaxBatch * kJobBytes) {
+        std::memcpy(ctx->h_jobs, host_blk.data(), blk_bytes);
+        std::memcpy((char*)ctx->h_jobs + blk_bytes, &P, sizeof P);
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, ctx->h_jobs, blk_bytes + sizeof P, hipMemcpyHostToDevice,
+                                          ctx->stream));
+    } else {
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, host_blk.data(), blk_bytes, hipMemcpyHostToDevice, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+    }
 
     unsigned int* sync_words =
         (unsigned int*)(((uintptr_t)((char*)Pdev + sizeof(LsmParams)) + 255) & ~(uintptr_t)255);
@@ -1208,24 +1243,48 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
                             "persistent LSM form does not fit N=%lld (%lld blocks of %d, at most %d co-resident)",
                             (long long)N, (long long)g, bs, gmax);
     }
+    // Read-back of the per-date flags and the persistent form's fin/err words,
+    // staged in the context's pinned buffer (after out[0..3]) so the one
+    // synchronisation in pcf_timing_end covers every copy.
+    const size_t ncoef = 4 * (size_t)(M + 1);
+    const bool coef_pinned = 8 + ncoef <= (size_t)kOutDoubles;
+    unsigned int* const sync_pin = reinterpret_cast<unsigned int*>(ctx->h_out + 4);
+    sync_pin[0] = sync_pin[1] = sync_pin[2] = sync_pin[3] = 0u;
+    auto stage_readback = [&]() -> int {
+        if (coef_pinned)
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_out + 8, cbase, ncoef * sizeof(double), hipMemcpyDeviceToHost,
+                                              ctx->stream));
+        if (persistent)
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_pin, sync_words + kArrivalCounters * kCounterStride,
+                                              4 * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+        return PCF_OK;
+    };
     if (persistent) {
         PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * (2 * 8 * sizeof(double) + 4 * sizeof(pcf::CSum))));
         b.partials = ctx->partials.as<pcf::CSum>();
         const int fin_w = kArrivalCounters * kCounterStride;
         LsmSync sy{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
         PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, sizeof(unsigned int) * kSyncWords, ctx->stream));
+        hipStream_t cs = nullptr;
+        PCF_HIP_CHECK(ctx, coop_stream(ctx->device, &cs));
+        // inputs ready on the context's stream -> the launch on the shared
+        // stream -> results read back on the context's stream
+        PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ready, ctx->stream));
         {
             std::lock_guard<std::mutex> pl(persistent_mu());
-            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+            PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
+            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, cs));
             hipError_t le;
-            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
-            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
-            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
-            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, ctx->stream);
-            if (le != hipSuccess) return pcf_hip_fail(ctx, le, "hipLaunchCooperativeKernel (LSM)");
-            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
-            if ((st = pcf_timing_end(ctx, 4, res))) return st;
+            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, cs);
+            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, cs);
+            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, cs);
+            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, cs);
+            if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
+            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, cs));
         }
+        PCF_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_kend, 0));
+        if ((st = stage_readback())) return st;
+        if ((st = pcf_timing_end(ctx, 4, res))) return st;
         res->grid_blocks = pgrid;
         res->block_size = PB;
     }
@@ -1273,29 +1332,33 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
         PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+        if ((st = stage_readback())) return st;
         if ((st = pcf_timing_end(ctx, 4, res))) return st;
         res->grid_blocks = grid_step;
         res->block_size = B;
     }
     const double h0 = ctx->h_out[0], h1 = ctx->h_out[1], h2 = ctx->h_out[2], h3 = ctx->h_out[3];
 
-    // Per-date flags.
-    std::vector<double> coefs(4 * (size_t)(M + 1));
-    PCF_HIP_CHECK(ctx, hipMemcpyAsync(coefs.data(), cbase, coefs.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                      ctx->stream));
-    if (o.when_out)
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(o.when_out, b.when, (size_t)N * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                          ctx->stream));
-    unsigned int sync_host[4] = {0, 0, 0, 0};  // fin, err
-    if (persistent)
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_host, sync_words + kArrivalCounters * kCounterStride, sizeof sync_host,
-                                          hipMemcpyDeviceToHost, ctx->stream));
-    PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (sync_host[1])
+    // Per-date flags (beyond the pinned staging area: one more copy).
+    std::vector<double> coefs;
+    const double* cf_host = ctx->h_out + 8;
+    if (!coef_pinned || o.when_out) {
+        if (!coef_pinned) {
+            coefs.resize(ncoef);
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(coefs.data(), cbase, ncoef * sizeof(double), hipMemcpyDeviceToHost,
+                                              ctx->stream));
+            cf_host = coefs.data();
+        }
+        if (o.when_out)
+            PCF_HIP_CHECK(ctx, hipMemcpyAsync(o.when_out, b.when, (size_t)N * sizeof(int32_t),
+                                              hipMemcpyDeviceToHost, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (persistent && sync_pin[1])
         return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
     int small = 0, fallback = 0, singular = 0;
     for (int m = 1; m <= M - 1; ++m) {
-        const int f = (int)coefs[4 * m];
+        const int f = (int)cf_host[4 * m];
         small += f == kFlagSmall;
         fallback += f == kFlagReduced;
         singular += f == kFlagSingular;

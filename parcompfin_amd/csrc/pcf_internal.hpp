@@ -74,6 +74,7 @@ struct pcf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_kbeg = nullptr, ev_kend = nullptr, ev_end = nullptr;
+    hipEvent_t ev_ready = nullptr;  // hand-off to another stream (no timing)
     std::string last_error;
     std::recursive_mutex mu;
     int cu_count = 0;
