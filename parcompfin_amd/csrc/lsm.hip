@@ -692,6 +692,50 @@ __device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict
     for (int k = 0; k < 8; ++k) g[k] = tot_sh[k];
 }
 
+// Row staging in LDS.  With kLdsRow, the path values of two dates live in the
+// block's dynamic LDS, [2 date parities][2 * PMAX slots][threads], and the
+// next date's row is copied there straight from HBM (global_load_lds, no
+// registers) while the current date is processed.  Each wave copies exactly
+// the 64 entries of every slot its own threads read (4-byte units: 1 copy per
+// slot for FP32, 2 for FP64), so a wave only waits for its own copies.
+template <typename Real, int PMAX>
+constexpr bool lsm_lds_row() {
+    return 2 * PMAX * sizeof(Real) <= 64;  // two parities x 1024 threads <= 128 KB
+}
+template <typename Real, int PMAX>
+constexpr size_t lsm_lds_row_bytes(int threads) {
+    return lsm_lds_row<Real, PMAX>() ? (size_t)2 * 2 * PMAX * threads * sizeof(Real) : 0;
+}
+
+// Copies the owned values of date m (store row m-1) into rowbuf parity m & 1.
+// Lanes whose pair index is past N/2 copy nothing (their entries are never
+// read), so every load stays inside the row.
+template <typename Real, int PMAX>
+__device__ __forceinline__ void lsm_stage_row(Real* rowbuf, const Real* __restrict__ store, int m, int64_t N,
+                                              int64_t half, int64_t T) {
+    constexpr int kUnits = (int)(sizeof(Real) / 4);  // 4-byte copies per entry
+    const unsigned B = blockDim.x;
+    const int wave = threadIdx.x / pcf::kWave, lane = threadIdx.x & (pcf::kWave - 1);
+    Real* const buf = rowbuf + (size_t)(m & 1) * 2 * PMAX * B;
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        const int64_t nw0 = (int64_t)blockIdx.x * B + (int64_t)k * T + (int64_t)wave * pcf::kWave;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const char* g = (const char*)(store + (int64_t)(m - 1) * N + nw0 + h * half);
+            char* l = (char*)(buf + (size_t)(2 * k + h) * B + (size_t)wave * pcf::kWave);
+#pragma unroll
+            for (int j = 0; j < kUnits; ++j) {
+                const int unit = j * pcf::kWave + lane;  // 4-byte unit of this wave's 64 entries
+                if (nw0 + unit / kUnits < half)
+                    __builtin_amdgcn_global_load_lds((const void*)(g + 4 * unit),
+                                                     (__attribute__((address_space(3))) void*)(l + 4 * j * pcf::kWave),
+                                                     4, 0, 0);
+            }
+        }
+    }
+}
+
 // Discount factors exp(-r dt k), k = 0..M, live in LDS (the persistent form
 // needs M + 1 <= kDiscLds; longer schedules run as the graph of launches).
 constexpr int kDiscLds = 1024;
@@ -702,7 +746,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     __shared__ double ldsp[16 * 8];
     __shared__ double red_a[16 * 8], red_b[16 * 8], tot_sh[8];
     __shared__ double disc[kDiscLds];
-    const LsmParams& P = *b.P;
+    extern __shared__ __align__(16) unsigned char lsm_dyn_lds[];  // row staging (kLdsRow)
+    // A private copy: the row copies into LDS count as memory writes, and
+    // through a reference every use would reload (and wait for) the block.
+    const LsmParams P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
@@ -768,13 +815,18 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             stb[2 * k + 1] = cf[2 * k + 1];
         }
     }
-    // Backward sweep: one grid all-reduce per exercise date.  Sc holds the
-    // owned paths' values at date m.  While a block waits for the others at
-    // date m it loads the row of date m-1 (into Sn; at 8 pairs per thread the
-    // registers would spill, so the row is read twice there) and forms that
-    // date's x-moments xm, which do not depend on the decisions at m.
-    constexpr bool kPrefetch = PMAX <= 4;
-    Real Sc[2 * PMAX], Sn[kPrefetch ? 2 * PMAX : 1];
+    // Backward sweep: one grid all-reduce per exercise date.  While a block
+    // waits for the others at date m it forms the x-moments xm of date m-1,
+    // which do not depend on the decisions at m.  The path values of a date
+    // come from the LDS row staging (kLdsRow: the row of date m-1 is copied in
+    // at the top of date m) or, where two staged rows do not fit, from
+    // registers: Sc holds date m and date m-1 is loaded into Sn after the
+    // block's arrival (at 8 pairs per thread the row of m-1 is loaded into
+    // transient registers instead, Sc being reloaded at the next date).
+    constexpr bool kLdsRow = lsm_lds_row<Real, PMAX>();
+    constexpr bool kPrefetch = !kLdsRow && PMAX <= 4;
+    Real* const rowbuf = reinterpret_cast<Real*>(lsm_dyn_lds);
+    Real Sc[kLdsRow ? 1 : 2 * PMAX], Sn[kPrefetch ? 2 * PMAX : 1];
     auto load_row = [&](int m, Real* dst) {
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
@@ -784,45 +836,63 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             dst[2 * k + 1] = own ? store[(int64_t)(m - 1) * N + n + half] : Real(0);
         }
     };
-    auto x_moments = [&](const Real* row, double xm[5]) {
+    // Value of slot i = 2k + h at date m from the staging or from `row`.
+    auto sval = [&](const Real* row, int m, int i) -> Real {
+        if constexpr (kLdsRow)
+            return rowbuf[((size_t)(m & 1) * 2 * PMAX + i) * blockDim.x + threadIdx.x];
+        return row[i];
+    };
+    auto x_moments = [&](const Real* row, int m, double xm[5]) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) xm[k] = 0.0;
 #pragma unroll
         for (int k = 0; k < PMAX; ++k)
             if (t0 + (int64_t)k * T < half) {
-                lsm_accumulate_x<Real, REF>(P, row[2 * k], xm);
-                lsm_accumulate_x<Real, REF>(P, row[2 * k + 1], xm);
+                lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k), xm);
+                lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k + 1), xm);
             }
     };
     double xm[5] = {0, 0, 0, 0, 0};
     if (M >= 2) {
-        load_row(M - 1, Sc);
-        x_moments(Sc, xm);
+        if constexpr (kLdsRow) {
+            lsm_stage_row<Real, PMAX>(rowbuf, store, M - 1, N, half, T);
+            pcf::stores_complete();  // this wave's copies have landed
+        } else {
+            load_row(M - 1, Sc);
+        }
+        x_moments(Sc, M - 1, xm);
     }
     unsigned int gen = 0;
     for (int m = M - 1; m >= 1; --m) {
         ++gen;
-        if constexpr (!kPrefetch)
+        if constexpr (kLdsRow) {
+            if (m > 1) lsm_stage_row<Real, PMAX>(rowbuf, store, m - 1, N, half, T);
+        } else if constexpr (!kPrefetch) {
             if (m != M - 1) load_row(m, Sc);
+        }
         double acc[8] = {xm[0], xm[1], xm[2], xm[3], xm[4], 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
             const int64_t n = t0 + (int64_t)k * T;
             if (n < half) {
-                lsm_accumulate_y<Real, REF>(P, disc, m, Sc[2 * k], when[2 * k], cf[2 * k], acc);
-                lsm_accumulate_y<Real, REF>(P, disc, m, Sc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
+                lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc);
+                lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1], cf[2 * k + 1],
+                                            acc);
             }
         }
         double* const slots_m = slots + (size_t)(m & 1) * gridDim.x * 8;
         lsm_publish(block_sum8(acc, red_a), slots_m, sy);
         if (m > 1) {
-            if constexpr (kPrefetch) {
+            if constexpr (kLdsRow) {
+                pcf::stores_complete();  // this wave's copies of date m-1 have landed
+                x_moments(Sc, m - 1, xm);
+            } else if constexpr (kPrefetch) {
                 load_row(m - 1, Sn);
-                x_moments(Sn, xm);
+                x_moments(Sn, m - 1, xm);
             } else {
                 Real Sx[2 * PMAX];
                 load_row(m - 1, Sx);
-                x_moments(Sx, xm);
+                x_moments(Sx, m - 1, xm);
             }
         }
         lsm_wait_all(sy, gen);
@@ -841,7 +911,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
                     for (int h = 0; h < 2; ++h) {
                         const int i = 2 * k + h;
                         Real stv = REF ? stb[REF ? i : 0] : cf[i];
-                        (void)lsm_decide<Real, REF>(P, disc, cm, m, Sc[i], when[i], cf[i], stv);
+                        (void)lsm_decide<Real, REF>(P, disc, cm, m, sval(Sc, m, i), when[i], cf[i], stv);
                         if constexpr (REF) stb[i] = stv;
                     }
                 }
@@ -852,7 +922,8 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             for (int i = 0; i < 2 * PMAX; ++i) Sc[i] = Sn[i];
         // (tot_sh and the reduction buffers are rewritten only after the
         // barriers of the next date, which every thread reaches after this
-        // date's reads)
+        // date's reads; a staging parity is rewritten one date after its last
+        // read, by the same wave)
     }
     // Final: discounted booked cash flows + European control.
     double s = 0.0, s2 = 0.0;
@@ -888,8 +959,34 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     }
 }
 
+// LDS the persistent kernel reserves beyond its own: the rest of the CU's LDS,
+// so that no other kernel's block shares a CU with it (a co-running block
+// slows the CU's share of every date, and every date waits for the slowest
+// block).  0 if the attributes cannot be read.
 template <typename Real, int SRC, bool REF, int PMAX>
-hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, int B, hipStream_t s) {
+size_t lsm_reserve_lds(int device) {
+    static size_t cached[64];
+    static bool have[64];
+    static std::mutex mu;
+    if (device < 0 || device >= 64) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!have[device]) {
+        hipFuncAttributes fa{};
+        int lds_cu = 0;
+        size_t r = 0;
+        if (hipFuncGetAttributes(&fa, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>) == hipSuccess &&
+            hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) ==
+                hipSuccess &&
+            (size_t)lds_cu > fa.sharedSizeBytes + 1024)
+            r = ((size_t)lds_cu - fa.sharedSizeBytes - 1024) & ~(size_t)1023;
+        cached[device] = r;
+        have[device] = true;
+    }
+    return cached[device];
+}
+
+template <typename Real, int SRC, bool REF, int PMAX>
+hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, int B, hipStream_t s, int device) {
     LsmBufs bb = b;
     LsmSync ss = sy;
     void* args[] = {&bb, &ss};
@@ -899,14 +996,14 @@ hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, in
     // so all blocks become resident -- at worst after a co-running kernel
     // ends.  Every wait is bounded all the same.
     return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
-                           dim3((unsigned)B), args, 0, s);
+                           dim3((unsigned)B), args, lsm_reserve_lds<Real, SRC, REF, PMAX>(device), s);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX>
-int max_coresident_t(int B, int cus) {
+int max_coresident_t(int B, int cus, int device) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, B,
-                                                     0) != hipSuccess)
+                                                     lsm_reserve_lds<Real, SRC, REF, PMAX>(device)) != hipSuccess)
         return 0;
     return nb * cus;
 }
@@ -914,23 +1011,36 @@ int max_coresident_t(int B, int cus) {
 // Dispatch over (Real, SRC, REF) and the register budget PMAX in {1,2,4,8}.
 template <int PMAX>
 hipError_t launch_persistent(bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid, int B,
-                             hipStream_t s) {
-    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX>(b, sy, grid, B, s);
+                             hipStream_t s, int dev) {
+    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
     if (src == kHost)
-        return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s)
-                   : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s);
-    return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s)
-               : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s);
+        return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s, dev)
+                   : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s, dev);
+    return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s, dev)
+               : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
+}
+
+// True when the LDS the launch reserves holds the kernel's row staging.
+template <typename Real, int SRC, bool REF, int PMAX>
+bool lds_fits_t(int B, int device) {
+    return lsm_lds_row_bytes<Real, PMAX>(B) <= lsm_reserve_lds<Real, SRC, REF, PMAX>(device);
+}
+template <int PMAX>
+bool lds_fits(bool f64, int src, bool ref, int B, int dev) {
+    if (!f64) return lds_fits_t<float, kPhilox, false, PMAX>(B, dev);
+    if (src == kHost)
+        return ref ? lds_fits_t<double, kHost, true, PMAX>(B, dev) : lds_fits_t<double, kHost, false, PMAX>(B, dev);
+    return ref ? lds_fits_t<double, kPhilox, true, PMAX>(B, dev) : lds_fits_t<double, kPhilox, false, PMAX>(B, dev);
 }
 
 template <int PMAX>
-int max_coresident(bool f64, int src, bool ref, int B, int cus) {
-    if (!f64) return max_coresident_t<float, kPhilox, false, PMAX>(B, cus);
+int max_coresident(bool f64, int src, bool ref, int B, int cus, int dev) {
+    if (!f64) return max_coresident_t<float, kPhilox, false, PMAX>(B, cus, dev);
     if (src == kHost)
-        return ref ? max_coresident_t<double, kHost, true, PMAX>(B, cus)
-                   : max_coresident_t<double, kHost, false, PMAX>(B, cus);
-    return ref ? max_coresident_t<double, kPhilox, true, PMAX>(B, cus)
-               : max_coresident_t<double, kPhilox, false, PMAX>(B, cus);
+        return ref ? max_coresident_t<double, kHost, true, PMAX>(B, cus, dev)
+                   : max_coresident_t<double, kHost, false, PMAX>(B, cus, dev);
+    return ref ? max_coresident_t<double, kPhilox, true, PMAX>(B, cus, dev)
+               : max_coresident_t<double, kPhilox, false, PMAX>(B, cus, dev);
 }
 
 // Persistent LSM kernels run one at a time per device (two of them could each
@@ -1228,10 +1338,16 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         while (p2 < pm) p2 <<= 1;
         const int64_t g = (P.half + (int64_t)p2 * bs - 1) / ((int64_t)p2 * bs);
         int gmax;
-        if (p2 == 1) gmax = max_coresident<1>(f64, src, ref, bs, ctx->cu_count);
-        else if (p2 == 2) gmax = max_coresident<2>(f64, src, ref, bs, ctx->cu_count);
-        else if (p2 == 4) gmax = max_coresident<4>(f64, src, ref, bs, ctx->cu_count);
-        else gmax = max_coresident<8>(f64, src, ref, bs, ctx->cu_count);
+        if (p2 == 1) gmax = max_coresident<1>(f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 2) gmax = max_coresident<2>(f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 4) gmax = max_coresident<4>(f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else gmax = max_coresident<8>(f64, src, ref, bs, ctx->cu_count, ctx->device);
+        bool lds_ok;
+        if (p2 == 1) lds_ok = lds_fits<1>(f64, src, ref, bs, ctx->device);
+        else if (p2 == 2) lds_ok = lds_fits<2>(f64, src, ref, bs, ctx->device);
+        else if (p2 == 4) lds_ok = lds_fits<4>(f64, src, ref, bs, ctx->device);
+        else lds_ok = lds_fits<8>(f64, src, ref, bs, ctx->device);
+        if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
         if (g >= 1 && g <= gmax) {
             persistent = true;
             pgrid = (int)g;
@@ -1275,10 +1391,10 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, cs));
             hipError_t le;
-            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, cs);
-            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, cs);
-            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, cs);
-            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, cs);
+            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, cs));
         }
