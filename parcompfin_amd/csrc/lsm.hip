@@ -679,14 +679,28 @@ __device__ __forceinline__ void lsm_wait_all(const LsmSync& s, unsigned int gen)
 
 __device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict__ slots, double* red_b,
                                            double* tot_sh) {
-    double t[8];
+    // Only the first min(grid, block) threads fetch; the waves holding them
+    // sum in the fixed order of block_sum8 (wave trees, then waves in order),
+    // the others stay out.
+    const unsigned int active = gridDim.x < blockDim.x ? gridDim.x : blockDim.x;
+    const int nw = (int)((active + pcf::kWave - 1) / pcf::kWave);
+    const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
+    if (wave < nw) {
+        double t[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) t[k] = 0.0;
-    for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+        for (int k = 0; k < 8; ++k) t[k] = 0.0;
+        for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
-    const double tot = block_sum8(t, red_b);
-    if (threadIdx.x < 8) tot_sh[threadIdx.x] = tot;
+            for (int k = 0; k < 8; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
+        const double d = wave_sum8(t);
+        if ((lane & 7) == 0) red_b[wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double tot = red_b[threadIdx.x];
+        for (int w = 1; w < nw; ++w) tot += red_b[w * 8 + threadIdx.x];
+        tot_sh[threadIdx.x] = tot;
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 8; ++k) g[k] = tot_sh[k];
