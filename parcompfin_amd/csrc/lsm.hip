@@ -973,30 +973,11 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     }
 }
 
-// LDS the persistent kernel reserves beyond its own: the rest of the CU's LDS,
-// so that no other kernel's block shares a CU with it (a co-running block
-// slows the CU's share of every date, and every date waits for the slowest
-// block).  0 if the attributes cannot be read.
+// Dynamic LDS of a persistent launch: the row staging only (kLdsRow), so that
+// a block can still be placed on a CU that runs other kernels' blocks.
 template <typename Real, int SRC, bool REF, int PMAX>
-size_t lsm_reserve_lds(int device) {
-    static size_t cached[64];
-    static bool have[64];
-    static std::mutex mu;
-    if (device < 0 || device >= 64) return 0;
-    std::lock_guard<std::mutex> lk(mu);
-    if (!have[device]) {
-        hipFuncAttributes fa{};
-        int lds_cu = 0;
-        size_t r = 0;
-        if (hipFuncGetAttributes(&fa, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>) == hipSuccess &&
-            hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) ==
-                hipSuccess &&
-            (size_t)lds_cu > fa.sharedSizeBytes + 1024)
-            r = ((size_t)lds_cu - fa.sharedSizeBytes - 1024) & ~(size_t)1023;
-        cached[device] = r;
-        have[device] = true;
-    }
-    return cached[device];
+size_t lsm_dyn_lds_bytes(int threads) {
+    return lsm_lds_row_bytes<Real, PMAX>(threads);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX>
@@ -1010,14 +991,14 @@ hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, in
     // so all blocks become resident -- at worst after a co-running kernel
     // ends.  Every wait is bounded all the same.
     return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
-                           dim3((unsigned)B), args, lsm_reserve_lds<Real, SRC, REF, PMAX>(device), s);
+                           dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX>
 int max_coresident_t(int B, int cus, int device) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, B,
-                                                     lsm_reserve_lds<Real, SRC, REF, PMAX>(device)) != hipSuccess)
+                                                     lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B)) != hipSuccess)
         return 0;
     return nb * cus;
 }
@@ -1034,10 +1015,25 @@ hipError_t launch_persistent(bool f64, int src, bool ref, const LsmBufs& b, cons
                : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
 }
 
-// True when the LDS the launch reserves holds the kernel's row staging.
+// True when the kernel's static LDS plus its row staging fit one block.
 template <typename Real, int SRC, bool REF, int PMAX>
 bool lds_fits_t(int B, int device) {
-    return lsm_lds_row_bytes<Real, PMAX>(B) <= lsm_reserve_lds<Real, SRC, REF, PMAX>(device);
+    static int lds_block[64];
+    static size_t lds_static[64];
+    static bool have[64];
+    static std::mutex mu;
+    if (device < 0 || device >= 64) return false;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!have[device]) {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_block[device], hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+                hipSuccess)
+            return false;
+        lds_static[device] = fa.sharedSizeBytes;
+        have[device] = true;
+    }
+    return lds_static[device] + lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B) <= (size_t)lds_block[device];
 }
 template <int PMAX>
 bool lds_fits(bool f64, int src, bool ref, int B, int dev) {
