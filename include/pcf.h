@@ -164,6 +164,16 @@ int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
                 int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
                 pcf_result* res);
 
+/* Backward induction on the same beta tree (the reference's lattice programs):
+ * american = 0 replaces double binom(S0,E,r,sigma,T,N,payoff_fun) of
+ * src/binom_vanilla_eur.cpp:5-39, american = 1 the one of
+ * src/binom_vanilla_amer.cpp:5-40.  FP64, the reference's operations and
+ * glibc powers, so the result equals the reference program's bit for bit.
+ * N <= 2^21. */
+int pcf_binom_vanilla(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                      int64_t N, double payoff_fun, int32_t american, const pcf_opts* opts,
+                      pcf_result* res);
+
 /* Optional per-asset extension of the basket (NULL pointers = the reference's
  * scalar sigma, S0, weights 1/d and equicorrelation rho). */
 typedef struct pcf_basket_ext {

@@ -154,6 +154,16 @@ class Engine:
         self._check(st)
         return Result.from_c(res)
 
+    def binom_vanilla(self, S0, E, r, sigma, T, N, payoff_fun, american=False, **kw) -> Result:
+        """The reference's lattice programs (src/binom_vanilla_eur.cpp:5-39,
+        src/binom_vanilla_amer.cpp:5-40): backward induction, FP64, bit-equal."""
+        o, keep = self.make_opts(**kw)
+        res = N_.PcfResult()
+        st = self.lib.pcf_binom_vanilla(self.ctx, S0, E, r, sigma, T, int(N), payoff_code(payoff_fun),
+                                        1 if american else 0, C.byref(o), C.byref(res))
+        self._check(st)
+        return Result.from_c(res)
+
     def mc_asia(self, S0, E, r, sigma, T, N, M, payoff_fun, **kw) -> Result:
         o, keep = self.make_opts(**kw)
         res = N_.PcfResult()

@@ -86,7 +86,7 @@ int pcf_ctx_destroy(pcf_ctx* ctx) {
     pcf_lsm_release(ctx);
     for (DevBuf* b : {&ctx->partials, &ctx->counters, &ctx->out, &ctx->normals, &ctx->logtab,
                       &ctx->prefix, &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef,
-                      &ctx->basket_L, &ctx->scratch, &ctx->jobs})
+                      &ctx->basket_L, &ctx->scratch, &ctx->jobs, &ctx->lattice})
         b->release();
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     if (ctx->h_jobs) (void)hipHostFree(ctx->h_jobs);

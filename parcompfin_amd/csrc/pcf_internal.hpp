@@ -95,6 +95,9 @@ struct pcf_ctx {
     DevBuf basket_L;   // basket Cholesky factor + per-asset constants
     DevBuf scratch;    // misc (normals dump)
     DevBuf jobs;       // job records of the current batched launch
+    DevBuf lattice;    // binomial lattice: u^i, d^k tables + two level rows
+    uint64_t lattice_key[3] = {0, 0, 0};  // (u, d, N) the tables hold
+    void* lattice_ptr = nullptr;          // where they were written
     BasketCache basket_cache;
     double* h_out = nullptr;  // pinned mirror of `out`
     void* h_jobs = nullptr;   // pinned staging of `jobs`

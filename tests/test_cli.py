@@ -1,0 +1,119 @@
+"""The drop-in programs (parcompfin_amd/bin/*): reference argv, reference row.
+
+CPU tests cover argument handling (it runs before any device is opened); the
+GPU tests run each program in verification mode (--rng mt19937 --seed S, the
+reference's own libstdc++ normal stream) and compare the printed row with the
+row the reference program printed for the same seed (tests/golden/ref_rows.csv,
+made with time() pinned): the Result, Abs_Error and Error columns must be the
+same text; Method, Parallel and the two time columns differ by design.
+"""
+import csv
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "parcompfin_amd", "bin")
+HERE = os.path.dirname(os.path.abspath(__file__))
+COLUMNS = ["Method", "Payoff", "S0", "E", "r", "sigma", "T", "N", "M", "Parallel", "Nr_of_assets", "T_overall",
+           "T_calculation", "Result", "Abs_Error", "Error"]
+
+
+def run(prog, *args, env=None):
+    return subprocess.run([os.path.join(BIN, prog)] + [str(a) for a in args], capture_output=True, text=True,
+                          timeout=120, env=env)
+
+
+PROGS = ["mc_eur", "binom_embar", "mc_asia", "mc_amer", "mc_eur_multi", "binom_vanilla_eur", "binom_vanilla_amer"]
+
+
+@pytest.mark.parametrize("prog", PROGS)
+def test_programs_built(prog):
+    assert os.access(os.path.join(BIN, prog), os.X_OK), "run __graft_entry__.build() first"
+
+
+@pytest.mark.parametrize("prog,extra", [("mc_eur", []), ("mc_asia", [252]), ("mc_amer", [50]),
+                                        ("mc_eur_multi", [8, 0.3]), ("binom_embar", []),
+                                        ("binom_vanilla_amer", [])])
+def test_unknown_payoff_is_rejected_like_the_reference(prog, extra):
+    # src/mc_eur.cpp:39-42 throws std::invalid_argument("Unknown payoff function").
+    p = run(prog, "straddle", 100, 100, 0.05, 0.2, 1, 1000, *extra)
+    assert p.returncode == 1 and "Unknown payoff function" in p.stderr
+
+
+def test_usage_and_bad_numbers():
+    p = run("mc_eur", "call", 100, 100)
+    assert p.returncode == 1 and "usage" in p.stderr
+    p = run("mc_asia", "call", 100, 100, 0.05, "x", 1, 1000, 252)
+    assert p.returncode == 1 and "sigma" in p.stderr
+    p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 1000, "--precision", "fp16")
+    assert p.returncode == 1
+
+
+def parse_row(text):
+    lines = [l for l in text.strip().splitlines() if l.strip()]
+    assert len(lines) == 1, text
+    f = lines[0].split(",")
+    assert len(f) == 16, lines[0]
+    return dict(zip(COLUMNS, f))
+
+
+def ref_rows():
+    return list(csv.DictReader(open(os.path.join(HERE, "golden", "ref_rows.csv"))))
+
+
+@pytest.mark.gpu
+def test_verification_rows_match_the_reference_programs():
+    rows = ref_rows()
+    assert len(rows) >= 20
+    for r in rows:
+        prog = r["prog"]
+        args = [r["payoff"], r["S0"], r["E"], r["r"], r["sigma"], r["T"], r["N"]]
+        if prog in ("mc_asia", "mc_amer"):
+            args.append(r["M"])
+        flags = ["--rng", "mt19937", "--seed", r["seed"]]
+        if prog == "mc_amer":
+            flags.append("--ref-exact")  # the reference's payoff(S-E,E) exercise test
+        print("[case]", prog, *args, *flags, flush=True)
+        p = run(prog, *args, *flags)
+        assert p.returncode == 0, p.stderr
+        got = parse_row(p.stdout)
+        want = dict(zip(COLUMNS, r["row"].split(",")))
+        assert got["Method"] == "HIP_verify" and got["Parallel"] == "1"
+        for c in ("Payoff", "S0", "E", "r", "sigma", "T", "N", "M", "Nr_of_assets", "Result", "Abs_Error", "Error"):
+            assert got[c] == want[c], (prog, c, got[c], want[c])
+
+
+@pytest.mark.gpu
+def test_binomial_programs_print_the_reference_values():
+    fx = {(r["prog"], r["payoff"], int(r["N"]), r["E"]): float(r["result"])
+          for r in csv.DictReader(open(os.path.join(HERE, "golden", "ref_kernels.csv")))}
+    for prog, pf, N, E, r_, s in (("binom_embar", "call", 1000, "100", 0.05, 0.2),
+                                  ("binom_vanilla_eur", "call", 2000, "100", 0.05, 0.2),
+                                  ("binom_vanilla_amer", "put", 2000, "100", 0.06, 0.2)):
+        p = run(prog, pf, 100, E, r_, s, 1, N)
+        assert p.returncode == 0, p.stderr
+        got = parse_row(p.stdout)
+        assert got["Method"] in ("HIP_fp64", "HIP_vanilla")
+        assert got["Result"] == "%.10g" % fx[(prog, pf, N, E)], (prog, got["Result"])
+
+
+@pytest.mark.gpu
+def test_comparison_column_from_the_environment():
+    env = dict(os.environ, PCF_COMPARISON="10.45")
+    p = run("binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1, 1000, env=env)
+    assert p.returncode == 0, p.stderr
+    got = parse_row(p.stdout)
+    res = float(got["Result"])  # printed with 10 significant digits
+    assert abs(float(got["Error"]) - (res - 10.45)) < 1e-8
+    assert abs(float(got["Abs_Error"]) - abs(res - 10.45)) < 1e-8
+
+
+@pytest.mark.gpu
+def test_trailing_thread_count_is_accepted():
+    # the _omp programs' argv (src/mc_eur_omp.cpp:41): a trailing thread count
+    p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 100000, 16)
+    assert p.returncode == 0, p.stderr
+    got = parse_row(p.stdout)
+    assert got["Method"] == "HIP" and abs(float(got["Result"]) - 10.45) < 0.2
