@@ -191,11 +191,28 @@ def per_config(eng):
     summary("cfg3_mc_asia_2^22x252", r32, r64, 252,
             {"geometric_control_fp32k": r32.control_price, "geometric_closed_form": geo,
              "control_minus_closed_in_se": (r32.control_price - geo) / r32.control_se})
+    lat = eng.binom_vanilla(100, 100, 0.06, 0.2, 1.0, 2000, "put", american=True)
     r32, r64 = pair(lambda p: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=4, precision=p))
     summary("cfg4_mc_amer_put_2^20x50", r32, r64, 50,
             {"european_put": 5.166002511, "european_control_fp32k": r32.control_price,
-             "lattice_binom_vanilla_amer_N2000": 5.798877611783797,
+             "lattice_binom_vanilla_amer_N2000": lat.price,
+             "fp64_minus_lattice_in_se": (r64.price - lat.price) / r64.std_err,
              "hbm_gbs_fp32k": r32.bytes / (r32.kernel_ms * 1e-3) / 1e9})
+    # The reference's lattices on the GPU (src/binom_vanilla_{eur,amer}.cpp), at the
+    # sizes its runscripts use for the comparison column (runscript_mc_eur.sh:22,
+    # runscript_mc_amer.sh:23), each checked against the reference's own value.
+    for name, N, pf, r_, am, want in (("lattice_eur_call_N1e5", 100000, "call", 0.05, False, None),
+                                      ("lattice_amer_put_N1e4", 10000, "put", 0.06, True, 5.7989254457596333),
+                                      ("lattice_amer_put_N2000", 2000, "put", 0.06, True, 5.798877611783797)):
+        eng.binom_vanilla(100, 100, r_, 0.2, 1.0, N, pf, american=am)
+        lr = eng.binom_vanilla(100, 100, r_, 0.2, 1.0, N, pf, american=am)
+        d = {"price": lr.price, "kernel_ms": lr.kernel_ms, "launches": lr.grid_blocks,
+             "node_updates_per_s": 0.5 * (N + 1) * (N + 2) / (lr.kernel_ms * 1e-3)}
+        if want is not None:
+            d["reference"] = want
+            d["equal_to_reference"] = lr.price == want
+        out[name] = d
+        log(name, d)
     return out
 
 

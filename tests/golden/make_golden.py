@@ -40,9 +40,11 @@ for N in (1, 2, 3, 100, 101, 1000, 10000, 30000):
     CASES.append(("binom_embar", "call", 100, 100, 0.05, 0.2, 1, N, 0, 0))
 CASES.append(("binom_embar", "put", 100, 100, 0.05, 0.2, 1, 1000, 0, 0))
 CASES.append(("binom_embar", "call", 100, 110, 0.02, 0.75, 1, 1000, 0, 0))
-for N in (1000, 2000):
+for N in (1, 2, 3, 129, 640, 1000, 2000, 10000):
     CASES.append(("binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1, N, 0, 0))
     CASES.append(("binom_vanilla_amer", "put", 100, 100, 0.06, 0.2, 1, N, 0, 0))
+CASES.append(("binom_vanilla_amer", "call", 100, 110, 0.02, 0.75, 1, 1000, 0, 0))
+CASES.append(("binom_vanilla_eur", "put", 100, 110, 0.02, 0.75, 1, 1000, 0, 0))
 LONG = [("binom_embar", "call", 100, 100, 0.05, 0.2, 1, 100000, 0, 0)]
 
 
@@ -88,6 +90,10 @@ def main():
         for r in rows:
             w.writerow(r)
     cli_cases = [c for c in CASES if c[7] <= 100000 and c[0] in ("mc_eur", "mc_asia", "mc_amer", "binom_embar")][:24]
+    # deterministic programs: one row each
+    cli_cases += [("binom_embar", "call", 100, 100, 0.05, 0.2, 1, 1000, 0, 0),
+                  ("binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1, 2000, 0, 0),
+                  ("binom_vanilla_amer", "put", 100, 100, 0.06, 0.2, 1, 2000, 0, 0)]
     with open(os.path.join(HERE, "ref_rows.csv"), "w", newline="") as f:
         w = csv.writer(f)
         w.writerow(["prog", "payoff", "S0", "E", "r", "sigma", "T", "N", "M", "seed", "row"])

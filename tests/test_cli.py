@@ -72,7 +72,8 @@ def test_verification_rows_match_the_reference_programs():
         args = [r["payoff"], r["S0"], r["E"], r["r"], r["sigma"], r["T"], r["N"]]
         if prog in ("mc_asia", "mc_amer"):
             args.append(r["M"])
-        flags = ["--rng", "mt19937", "--seed", r["seed"]]
+        deterministic = prog.startswith("binom")  # no random stream: no flags
+        flags = [] if deterministic else ["--rng", "mt19937", "--seed", r["seed"]]
         if prog == "mc_amer":
             flags.append("--ref-exact")  # the reference's payoff(S-E,E) exercise test
         print("[case]", prog, *args, *flags, flush=True)
@@ -80,7 +81,8 @@ def test_verification_rows_match_the_reference_programs():
         assert p.returncode == 0, p.stderr
         got = parse_row(p.stdout)
         want = dict(zip(COLUMNS, r["row"].split(",")))
-        assert got["Method"] == "HIP_verify" and got["Parallel"] == "1"
+        method = {"binom_embar": "HIP_fp64"}.get(prog, "HIP_vanilla" if deterministic else "HIP_verify")
+        assert got["Method"] == method and got["Parallel"] == "1"
         for c in ("Payoff", "S0", "E", "r", "sigma", "T", "N", "M", "Nr_of_assets", "Result", "Abs_Error", "Error"):
             assert got[c] == want[c], (prog, c, got[c], want[c])
 
