@@ -125,6 +125,37 @@ def test_bad_request_is_isolated(engine):
         assert r.price == engine.mc_eur(100, 100, 0.05, 0.2, 1.0, 1 << 15, "call", seed=1).price
 
 
+@pytest.mark.skip(reason="batched persistent American launches are switched off: the first GPU run of that "
+                         "code passed every test, then the GPU was found faulted; the cause was not found, "
+                         "so the code was taken out and American requests run one launch each")
+def test_american_requests_share_persistent_launches(engine):
+    """American requests with one block size run in one persistent launch, each
+    on its own blocks, store and date barriers; the results are bitwise those of
+    the lone launches.  Eight 2^18-path requests exceed one co-resident grid,
+    so that unit is split over consecutive launches."""
+    cases = []
+    for s in range(6):  # 2^16 and 2^17 paths share a block size
+        cases.append(((100, 100, 0.06, 0.2, 1.0, (1 << 16) + 2 * s * 1000, "put"), dict(M=50, seed=100 + s)))
+        cases.append(((100, 100, 0.06, 0.2, 1.0, 1 << 17, "put"), dict(M=30, seed=200 + s)))
+    for s in range(4):
+        cases.append(((100, 110, 0.05, 0.3, 1.0, 1 << 16, "put"), dict(M=50, seed=300 + s, precision="fp64")))
+    for s in range(8):
+        cases.append(((100, 100, 0.06, 0.2, 1.0, 1 << 18, "put"), dict(M=20, seed=400 + s)))
+    say("american batch", len(cases))
+    alone = [engine.mc_amer(*a[:6], kw["M"], a[6], **{k: v for k, v in kw.items() if k != "M"}) for a, kw in cases]
+    with Server(engine, max_inflight=1) as srv:
+        tickets = [srv.submit("amer", *a, **kw) for a, kw in cases]
+        served = [srv.wait(t) for t in tickets]
+    for (a, kw), x, y in zip(cases, alone, served):
+        assert x.price == y.price and x.std_err == y.std_err, (a, kw, x.price, y.price)
+        assert x.control_price == y.control_price
+        assert (x.lsm_small_itm, x.lsm_fallback) == (y.lsm_small_itm, y.lsm_fallback)
+        assert (x.grid_blocks, x.block_size) == (y.grid_blocks, y.block_size)
+    sizes = [y.batch_size for y in served]
+    say("american batch sizes", sizes)
+    assert max(sizes) >= 4
+
+
 def test_served_fp32_within_se_of_fp64(engine):
     reqs = mixed_requests(5, 20, 16, 18)
     with Server(engine, max_inflight=4) as srv:
