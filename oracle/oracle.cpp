@@ -393,6 +393,134 @@ double orc_mc_amer_seed(double S0, double E, double r, double sigma, double T,
 }
 
 // ---------------------------------------------------------------------------
+// mc_amer with a K-term basis {1, x, ..., x^(K-1)}, x = S/E - 1 (K = 2..6): the
+// generalisation of the reference's regression (src/mc_amer.cpp:38-108, basis
+// {1, x, x^2}; the raw-S experiments in depr/mc_amer/v3/mc_amer.cpp:70-148)
+// with the production decision rule (corrected exercise test payoff(S, E),
+// continuation y = exp(-r dt (when - m)) * cash flow from a table).
+// Per date, over the in-the-money paths in path order:
+//   g[j]        = sum x^j,          j = 0..2K-2   (g[0] = count)
+//   g[2K-1 + i] = sum y x^i,        i = 0..K-1
+// then orc_lsm_solve_poly.  Fewer than 3 in-the-money paths: compare the
+// payoff with y directly.  Store: the reference recurrence on w (as
+// orc_mc_amer_z).
+// Pinned: K = 3 agrees with orc_mc_amer_z(quirk = 0) (the same fitted values
+// in exact arithmetic); the solve against numpy on random and exactly
+// polynomial systems; the whole estimator against an independent numpy
+// least-squares LSM (tests/test_oracle.py).
+//
+// Cholesky of the Hankel Gram G[i][j] = g[i+j]: column j is accepted while its
+// pivot p_j = G[j][j] - sum_t L[j][t]^2 exceeds 1e-13 G[j][j]; at the first
+// rejected column the leading kk x kk system is solved (kk < K: flag 4,
+// "reduced").  c = {flag, coefficients of x^0..x^(K-1)} (unused ones 0).
+// Flags: 0 no path in the money, 1 at most two, 2 full basis, 4 reduced.
+int orc_lsm_solve_poly(const double* g, int K, double* c) {
+    for (int i = 0; i <= K; ++i) c[i] = 0.0;
+    const double n = g[0];
+    if (n == 0.0) return (int)(c[0] = 0);
+    if (n <= 2.0) return (int)(c[0] = 1);
+    double L[8][8] = {{0}}, id[8] = {0}, y[8] = {0}, x[8] = {0};
+    int kk = K;
+    for (int j = 0; j < K; ++j) {
+        double p = g[2 * j];
+        for (int t = 0; t < j; ++t) p -= L[j][t] * L[j][t];
+        if (j > 0 && !(p > 1e-13 * g[2 * j])) {
+            kk = j;
+            break;
+        }
+        L[j][j] = sqrt(p);
+        id[j] = 1.0 / L[j][j];
+        for (int i = j + 1; i < K; ++i) {
+            double s = g[i + j];
+            for (int t = 0; t < j; ++t) s -= L[i][t] * L[j][t];
+            L[i][j] = s * id[j];
+        }
+    }
+    for (int i = 0; i < kk; ++i) {
+        double s = g[2 * K - 1 + i];
+        for (int t = 0; t < i; ++t) s -= L[i][t] * y[t];
+        y[i] = s * id[i];
+    }
+    for (int i = kk - 1; i >= 0; --i) {
+        double s = y[i];
+        for (int t = i + 1; t < kk; ++t) s -= L[t][i] * x[t];
+        x[i] = s * id[i];
+    }
+    for (int i = 0; i < kk; ++i) c[1 + i] = x[i];
+    c[0] = kk == K ? 2 : 4;
+    return (int)c[0];
+}
+
+double orc_mc_amer_poly(double S0, double E, double r, double sigma, double T, int64_t N, int M,
+                        double payoff_fun, const double* w, int K, int32_t* when_out, int32_t* flags_out) {
+    const int pf = (int)payoff_fun;
+    if (N % 2 != 0 || K < 2 || K > 6) return NAN;
+    const double dt = T / M, invE = 1.0 / E;
+    const int64_t half = N / 2;
+    std::vector<std::vector<double>> S(M + 1, std::vector<double>(N));
+    for (int64_t n = 0; n < half; ++n) {
+        S[0][n] = S0;
+        S[0][n + half] = S0;
+        for (int m = 1; m <= M; ++m) {
+            const double ww = w[n * (int64_t)M + (m - 1)];
+            S[m][n] = S[m - 1][n] * exp((r - 0.5 * sigma * sigma) * dt + sigma * ww);
+            S[m][n + half] = S[m - 1][n + half] * exp((r - 0.5 * sigma * sigma) * dt - sigma * ww);
+        }
+    }
+    std::vector<double> disc(M + 1);
+    for (int k = 0; k <= M; ++k) disc[k] = exp(-r * dt * k);
+    std::vector<int32_t> when(N, M);
+    std::vector<double> cf(N);
+    for (int64_t n = 0; n < N; ++n) cf[n] = payoff(S[M][n], E, pf);
+    for (int m = M - 1; m >= 1; --m) {
+        double g[20] = {0};
+        for (int64_t n = 0; n < N; ++n) {
+            if (!(payoff(S[m][n], E, pf) > 0.0)) continue;
+            const double x = S[m][n] * invE - 1.0;
+            const double y = disc[when[n] - m] * cf[n];
+            double p = x;
+            g[0] += 1.0;
+            g[1] += p;
+            for (int j = 2; j <= 2 * K - 2; ++j) {
+                p = p * x;
+                g[j] += p;
+            }
+            double q = y;
+            g[2 * K - 1] += q;
+            for (int i = 1; i < K; ++i) {
+                q = q * x;
+                g[2 * K - 1 + i] += q;
+            }
+        }
+        double c[8];
+        const int flag = orc_lsm_solve_poly(g, K, c);
+        if (flags_out) flags_out[m] = flag;
+        if (flag == 0) continue;
+        for (int64_t n = 0; n < N; ++n) {
+            const double pv = payoff(S[m][n], E, pf);
+            if (!(pv > 0.0)) continue;
+            double v;
+            if (flag == 1) {
+                v = disc[when[n] - m] * cf[n];
+            } else {
+                const double x = S[m][n] * invE - 1.0;
+                v = c[K];
+                for (int i = K - 1; i >= 1; --i) v = c[i] + x * v;
+            }
+            if (pv > v) {
+                when[n] = m;
+                cf[n] = pv;
+            }
+        }
+    }
+    double s = 0.0;
+    for (int64_t n = 0; n < N; ++n) s += disc[when[n]] * cf[n];
+    if (when_out)
+        for (int64_t n = 0; n < N; ++n) when_out[n] = when[n];
+    return std::max(payoff(S0, E, pf), s / (double)N);
+}
+
+// ---------------------------------------------------------------------------
 // Basket -- src/mc_eur_multi.cpp:6-35 with mvnorm, include/mvn.h:42-82.
 //   Z: standard normals, column-major d x N (asset fastest), the order in
 //   which mvnorm's NullaryExpr is filled.  L: lower Cholesky factor (d x d,

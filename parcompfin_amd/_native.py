@@ -59,7 +59,8 @@ class PcfOpts(C.Structure):
         ("items_per_thread", C.c_int32),
         ("when_out", C.POINTER(C.c_int32)),
         ("kernel_variant", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("lsm_basis", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

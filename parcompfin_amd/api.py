@@ -114,7 +114,7 @@ class Engine:
 
     @staticmethod
     def make_opts(seed=0x5EED, precision="fp32k", normals=None, ref_exact=False, control=False,
-                  block_size=0, items_per_thread=0, when_out=None, kernel_variant=0):
+                  block_size=0, items_per_thread=0, when_out=None, kernel_variant=0, lsm_basis=0):
         o = N_.PcfOpts()
         N_.load().pcf_opts_init(C.byref(o))
         o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
@@ -131,6 +131,7 @@ class Engine:
         o.block_size = int(block_size)
         o.items_per_thread = int(items_per_thread)
         o.kernel_variant = int(kernel_variant)
+        o.lsm_basis = int(lsm_basis)
         if when_out is not None:
             keep.append(when_out)
             o.when_out = when_out.ctypes.data_as(C.POINTER(C.c_int32))

@@ -280,6 +280,139 @@ __device__ __forceinline__ bool lsm_decide(const LsmParams& P, const double* __r
     return false;
 }
 
+// ---- K-term basis {1, x, ..., x^(K-1)}, x = S/E - 1 (K = 2, 4, 5, 6) --------
+// The quadratic regression above generalised to K terms (the reference
+// regresses on {1, x, x^2}, src/mc_amer.cpp:38-108; its raw-S experiments
+// depr/mc_amer/v3/mc_amer.cpp:70-148 keep three terms).  x is centred at the
+// strike so that the Hankel Gram stays usable in FP64 up to K = 6 (condition
+// ~5e9 for in-the-money puts, against ~4e11 on S/E).  Per date the sums are
+// the x-moments g[j] = sum x^j (j = 0..2K-2, g[0] = count) and the y-terms
+// g[2K-1+i] = sum y x^i (i = 0..K-1); they travel as chunks of 8 through the
+// same wave and grid reductions as the quadratic's 8 sums.  Production
+// arithmetic only (the reference-exact mode keeps the reference's basis).
+template <int K>
+struct LsmBasis {
+    static constexpr int kMom = 2 * K - 1;            // x-moments
+    static constexpr int kSums = 3 * K - 1;           // + y-terms
+    static constexpr int kChunks = (kSums + 7) / 8;   // reductions of 8
+    static constexpr int kCoef = K == 3 ? 4 : 8;      // per-date stride: flag + K coefficients
+};
+
+template <typename Real, int K>
+__device__ __forceinline__ void lsm_poly_x(const LsmParams& P, Real S, double* __restrict__ acc) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    if (pv > 0.0) {
+        const double x = (double)S * P.invE - 1.0;
+        double p = x;
+        acc[0] += 1.0;
+        acc[1] += p;
+#pragma unroll
+        for (int j = 2; j < 2 * K - 1; ++j) {
+            p = p * x;
+            acc[j] += p;
+        }
+    }
+}
+
+// y-terms into acc[0..K-1] (the caller passes &sums[2K-1]).
+template <typename Real, int K>
+__device__ __forceinline__ void lsm_poly_y(const LsmParams& P, const double* __restrict__ disc, int m, Real S,
+                                           int32_t when, Real cf, double* __restrict__ acc) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    if (pv > 0.0) {
+        const double x = (double)S * P.invE - 1.0;
+        double q = disc[when - m] * (double)cf;
+        acc[0] += q;
+#pragma unroll
+        for (int i = 1; i < K; ++i) {
+            q = q * x;
+            acc[i] += q;
+        }
+    }
+}
+
+// Normal equations of the K-term basis -> c = {flag, c_0 .. c_(K-1)}.
+// Cholesky of G[i][j] = g[i+j]; column j is accepted while its pivot exceeds
+// 1e-13 G[j][j], and at the first rejected column the leading kk x kk system
+// is solved (flag "reduced", the dropped coefficients 0).  A pure function of
+// g, evaluated redundantly by every thread.
+template <int K>
+__device__ __forceinline__ void lsm_solve_poly(const double* g, double c[K + 1]) {
+    const double n = g[0];
+#pragma unroll
+    for (int i = 1; i <= K; ++i) c[i] = 0.0;
+    if (n == 0.0) {
+        c[0] = kFlagSkip;
+        return;
+    }
+    if (n <= 2.0) {
+        c[0] = kFlagSmall;
+        return;
+    }
+    double L[K][K] = {}, id[K] = {}, y[K], x[K];
+    int kk = K;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double p = g[2 * j];
+#pragma unroll
+        for (int t = 0; t < j; ++t) p -= L[j][t] * L[j][t];
+        if (kk == K && j > 0 && !(p > 1e-13 * g[2 * j])) kk = j;
+        if (j < kk) {
+            L[j][j] = sqrt(p);
+            id[j] = 1.0 / L[j][j];
+#pragma unroll
+            for (int i = j + 1; i < K; ++i) {
+                double s = g[i + j];
+#pragma unroll
+                for (int t = 0; t < j; ++t) s -= L[i][t] * L[j][t];
+                L[i][j] = s * id[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        double s = g[2 * K - 1 + i];
+#pragma unroll
+        for (int t = 0; t < i; ++t) s -= L[i][t] * y[t];
+        y[i] = i < kk ? s * id[i] : 0.0;
+    }
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int t = i + 1; t < K; ++t) s -= L[t][i] * x[t];  // x[t] = 0 past kk
+        x[i] = i < kk ? s * id[i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) c[1 + i] = x[i];
+    c[0] = kk == K ? kFlagRegress : kFlagReduced;
+}
+
+// Exercise decision at date m with the K-term fit (Horner in x).
+template <typename Real, int K>
+__device__ __forceinline__ void lsm_poly_decide(const LsmParams& P, const double* __restrict__ disc,
+                                                const double* __restrict__ cm, int m, Real S, int32_t& when,
+                                                Real& cf) {
+    const int flag = (int)cm[0];
+    if (flag != kFlagSmall && flag != kFlagRegress && flag != kFlagReduced) return;
+    const Real pvr = pay<Real>(S, (Real)P.E, (Real)P.pf);
+    const double pv = (double)pvr;
+    if (!(pv > 0.0)) return;
+    double v;
+    if (flag == kFlagSmall) {
+        v = disc[when - m] * (double)cf;
+    } else {
+        const double x = (double)S * P.invE - 1.0;
+        v = cm[K];
+#pragma unroll
+        for (int i = K - 1; i >= 1; --i) v = cm[i] + x * v;
+    }
+    if (pv > v) {
+        when = m;
+        cf = pvr;
+    }
+}
+
 // --- k_lsm_paths ------------------------------------------------------------
 // Verification store: host normals w[n*M + (m-1)] (pathsfinder's draw order),
 // the reference's recurrence in FP64 without contraction.
@@ -706,6 +839,78 @@ __device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict
     for (int k = 0; k < 8; ++k) g[k] = tot_sh[k];
 }
 
+// The same three steps for 8C sums (C chunks of 8), used by the K-term basis:
+// chunk c of the block's sums is reduced by block_sum8's tree into its own LDS
+// rows (lds + 128 c), threads 0..7 then hold sum 8c + k in t[c]; the slots
+// hold 8C sums per block, sum-major.
+template <int C>
+__device__ __forceinline__ void block_sum_chunks(const double* v, double* lds, double t[C]) {
+    const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double d = wave_sum8(v + 8 * c);
+        if ((lane & 7) == 0)
+            lds[128 * c + wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
+    }
+    __syncthreads();
+    const int nw = (blockDim.x + pcf::kWave - 1) / pcf::kWave;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        t[c] = 0.0;
+        if (threadIdx.x < 8) {
+            t[c] = lds[128 * c + threadIdx.x];
+            for (int w = 1; w < nw; ++w) t[c] += lds[128 * c + w * 8 + threadIdx.x];
+        }
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void lsm_publish_chunks(const double t[C], double* __restrict__ slots, const LsmSync& s) {
+    if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
+        const int lane = threadIdx.x;
+        if (lane < 8)
+#pragma unroll
+            for (int c = 0; c < C; ++c) pcf::publish(slots + (size_t)(8 * c + lane) * gridDim.x + blockIdx.x, t[c]);
+        pcf::stores_complete();
+        if (lane == 0)
+            __hip_atomic_fetch_add(s.arrive + (blockIdx.x % kArrivalCounters) * kCounterStride, 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void lsm_gather_chunks(double* g, const double* __restrict__ slots, double* red_b,
+                                                  double* tot_sh) {
+    const unsigned int active = gridDim.x < blockDim.x ? gridDim.x : blockDim.x;
+    const int nw = (int)((active + pcf::kWave - 1) / pcf::kWave);
+    const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
+    if (wave < nw) {
+        double t[8 * C];
+#pragma unroll
+        for (int k = 0; k < 8 * C; ++k) t[k] = 0.0;
+        for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+#pragma unroll
+            for (int k = 0; k < 8 * C; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const double d = wave_sum8(t + 8 * c);
+            if ((lane & 7) == 0)
+                red_b[128 * c + wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8)
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            double tot = red_b[128 * c + threadIdx.x];
+            for (int w = 1; w < nw; ++w) tot += red_b[128 * c + w * 8 + threadIdx.x];
+            tot_sh[8 * c + threadIdx.x] = tot;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8 * C; ++k) g[k] = tot_sh[k];
+}
+
 // Row staging in LDS.  With kLdsRow, the path values of two dates live in the
 // block's dynamic LDS, [2 date parities][2 * PMAX slots][threads], and the
 // next date's row is copied there straight from HBM (global_load_lds, no
@@ -754,11 +959,15 @@ __device__ __forceinline__ void lsm_stage_row(Real* rowbuf, const Real* __restri
 // needs M + 1 <= kDiscLds; longer schedules run as the graph of launches).
 constexpr int kDiscLds = 1024;
 
-template <typename Real, int SRC, bool REF, int PMAX>
+// K: regression basis size (3: the quadratic of lsm_solve_g, else the K-term
+// basis of lsm_solve_poly).
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
 __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) {
+    using Bs = LsmBasis<K>;
+    static_assert(K == 3 || !REF, "the reference-exact mode keeps the reference's basis");
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
-    __shared__ double red_a[16 * 8], red_b[16 * 8], tot_sh[8];
+    __shared__ double red_a[16 * 8 * Bs::kChunks], red_b[16 * 8 * Bs::kChunks], tot_sh[8 * Bs::kChunks];
     __shared__ double disc[kDiscLds];
     extern __shared__ __align__(16) unsigned char lsm_dyn_lds[];  // row staging (kLdsRow)
     // A private copy: the row copies into LDS count as memory writes, and
@@ -770,7 +979,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int k = threadIdx.x; k <= M; k += blockDim.x) disc[k] = b.disc[k];
-    // Date slots: 2 parities x grid x 8 doubles, then the final sums' slots.
+    // Date slots: 2 parities x grid x 8 C doubles, then the final sums' slots.
     double* __restrict__ slots = (double*)b.partials;
     __syncthreads();
     // Forward: the owned pairs' paths into the store (runtime loop: no state yet).
@@ -856,17 +1065,22 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             return rowbuf[((size_t)(m & 1) * 2 * PMAX + i) * blockDim.x + threadIdx.x];
         return row[i];
     };
-    auto x_moments = [&](const Real* row, int m, double xm[5]) {
+    auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) xm[k] = 0.0;
+        for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
 #pragma unroll
         for (int k = 0; k < PMAX; ++k)
             if (t0 + (int64_t)k * T < half) {
-                lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k), xm);
-                lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k + 1), xm);
+                if constexpr (K == 3) {
+                    lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k), xm);
+                    lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k + 1), xm);
+                } else {
+                    lsm_poly_x<Real, K>(P, sval(row, m, 2 * k), xm);
+                    lsm_poly_x<Real, K>(P, sval(row, m, 2 * k + 1), xm);
+                }
             }
     };
-    double xm[5] = {0, 0, 0, 0, 0};
+    double xm[Bs::kMom] = {};
     if (M >= 2) {
         if constexpr (kLdsRow) {
             lsm_stage_row<Real, PMAX>(rowbuf, store, M - 1, N, half, T);
@@ -884,18 +1098,36 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
         } else if constexpr (!kPrefetch) {
             if (m != M - 1) load_row(m, Sc);
         }
-        double acc[8] = {xm[0], xm[1], xm[2], xm[3], xm[4], 0.0, 0.0, 0.0};
+        double* const slots_m = slots + (size_t)(m & 1) * gridDim.x * (8 * Bs::kChunks);
+        if constexpr (K == 3) {
+            double acc[8] = {xm[0], xm[1], xm[2], xm[3], xm[4], 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < PMAX; ++k) {
-            const int64_t n = t0 + (int64_t)k * T;
-            if (n < half) {
-                lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc);
-                lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1], cf[2 * k + 1],
-                                            acc);
+            for (int k = 0; k < PMAX; ++k) {
+                const int64_t n = t0 + (int64_t)k * T;
+                if (n < half) {
+                    lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc);
+                    lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1],
+                                                cf[2 * k + 1], acc);
+                }
             }
+            lsm_publish(block_sum8(acc, red_a), slots_m, sy);
+        } else {
+            double acc[8 * Bs::kChunks];
+#pragma unroll
+            for (int j = 0; j < 8 * Bs::kChunks; ++j) acc[j] = j < Bs::kMom ? xm[j] : 0.0;
+#pragma unroll
+            for (int k = 0; k < PMAX; ++k) {
+                const int64_t n = t0 + (int64_t)k * T;
+                if (n < half) {
+                    lsm_poly_y<Real, K>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc + Bs::kMom);
+                    lsm_poly_y<Real, K>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1], cf[2 * k + 1],
+                                        acc + Bs::kMom);
+                }
+            }
+            double t[Bs::kChunks];
+            block_sum_chunks<Bs::kChunks>(acc, red_a, t);
+            lsm_publish_chunks<Bs::kChunks>(t, slots_m, sy);
         }
-        double* const slots_m = slots + (size_t)(m & 1) * gridDim.x * 8;
-        lsm_publish(block_sum8(acc, red_a), slots_m, sy);
         if (m > 1) {
             if constexpr (kLdsRow) {
                 pcf::stores_complete();  // this wave's copies of date m-1 have landed
@@ -910,24 +1142,41 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             }
         }
         lsm_wait_all(sy, gen);
-        double g[8], cm[4];
-        lsm_gather(g, slots_m, red_b, tot_sh);
-        lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
-        if (blockIdx.x == 0 && threadIdx.x == 0)
+        if constexpr (K == 3) {
+            double g[8], cm[4];
+            lsm_gather(g, slots_m, red_b, tot_sh);
+            lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
+            if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
-        if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
+                for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
+            if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
+#pragma unroll
+                for (int k = 0; k < PMAX; ++k) {
+                    const int64_t n = t0 + (int64_t)k * T;
+                    if (n < half) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int i = 2 * k + h;
+                            Real stv = REF ? stb[REF ? i : 0] : cf[i];
+                            (void)lsm_decide<Real, REF>(P, disc, cm, m, sval(Sc, m, i), when[i], cf[i], stv);
+                            if constexpr (REF) stb[i] = stv;
+                        }
+                    }
+                }
+            }
+        } else {
+            double g[8 * Bs::kChunks], cm[K + 1];
+            lsm_gather_chunks<Bs::kChunks>(g, slots_m, red_b, tot_sh);
+            lsm_solve_poly<K>(g, cm);  // every thread, same totals -> same coefficients
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+#pragma unroll
+                for (int k = 0; k <= K; ++k) b.coef[Bs::kCoef * m + k] = cm[k];
 #pragma unroll
             for (int k = 0; k < PMAX; ++k) {
                 const int64_t n = t0 + (int64_t)k * T;
                 if (n < half) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int i = 2 * k + h;
-                        Real stv = REF ? stb[REF ? i : 0] : cf[i];
-                        (void)lsm_decide<Real, REF>(P, disc, cm, m, sval(Sc, m, i), when[i], cf[i], stv);
-                        if constexpr (REF) stb[i] = stv;
-                    }
+                    lsm_poly_decide<Real, K>(P, disc, cm, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k]);
+                    lsm_poly_decide<Real, K>(P, disc, cm, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1], cf[2 * k + 1]);
                 }
             }
         }
@@ -967,7 +1216,8 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
 #pragma unroll
     for (int k = 0; k < 4; ++k) a.v[k] = {threadIdx.x == 0 ? fin[k] : 0.0, 0.0};
     // Final sums: last-arriving block, on its own counter and slots.
-    if (lsm_arrive<4>(a, (pcf::CSum*)(slots + (size_t)2 * gridDim.x * 8), sy.fin, lds, tot) && threadIdx.x == 0) {
+    if (lsm_arrive<4>(a, (pcf::CSum*)(slots + (size_t)2 * gridDim.x * (8 * Bs::kChunks)), sy.fin, lds, tot) &&
+        threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) b.out[k] = tot.v[k].s + tot.v[k].c;
     }
@@ -980,7 +1230,7 @@ size_t lsm_dyn_lds_bytes(int threads) {
     return lsm_lds_row_bytes<Real, PMAX>(threads);
 }
 
-template <typename Real, int SRC, bool REF, int PMAX>
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
 hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, int B, hipStream_t s, int device) {
     LsmBufs bb = b;
     LsmSync ss = sy;
@@ -990,33 +1240,21 @@ hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, in
     // per device, and every other kernel of the library finishes on its own,
     // so all blocks become resident -- at worst after a co-running kernel
     // ends.  Every wait is bounded all the same.
-    return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, dim3((unsigned)grid),
+    return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, dim3((unsigned)grid),
                            dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s);
 }
 
-template <typename Real, int SRC, bool REF, int PMAX>
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
 int max_coresident_t(int B, int cus, int device) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>, B,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, B,
                                                      lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B)) != hipSuccess)
         return 0;
     return nb * cus;
 }
 
-// Dispatch over (Real, SRC, REF) and the register budget PMAX in {1,2,4,8}.
-template <int PMAX>
-hipError_t launch_persistent(bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid, int B,
-                             hipStream_t s, int dev) {
-    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
-    if (src == kHost)
-        return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s, dev)
-                   : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s, dev);
-    return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s, dev)
-               : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
-}
-
 // True when the kernel's static LDS plus its row staging fit one block.
-template <typename Real, int SRC, bool REF, int PMAX>
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
 bool lds_fits_t(int B, int device) {
     static int lds_block[64];
     static size_t lds_static[64];
@@ -1026,7 +1264,7 @@ bool lds_fits_t(int B, int device) {
     std::lock_guard<std::mutex> lk(mu);
     if (!have[device]) {
         hipFuncAttributes fa{};
-        if (hipFuncGetAttributes(&fa, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX>) != hipSuccess ||
+        if (hipFuncGetAttributes(&fa, (const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>) != hipSuccess ||
             hipDeviceGetAttribute(&lds_block[device], hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
                 hipSuccess)
             return false;
@@ -1035,22 +1273,88 @@ bool lds_fits_t(int B, int device) {
     }
     return lds_static[device] + lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B) <= (size_t)lds_block[device];
 }
+
+// Dispatch over (Real, SRC, REF), the register budget PMAX in {1,2,4,8} and
+// the basis size K: 3 (every mode), or 2, 4, 5, 6 (production arithmetic,
+// ref == false, checked by the caller).
+template <int PMAX, int K>
+hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid, int B,
+                               hipStream_t s, int dev) {
+    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(b, sy, grid, B, s, dev);
+    if constexpr (K == 3) {
+        if (src == kHost)
+            return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s, dev)
+                       : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s, dev);
+        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s, dev)
+                   : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
+    } else {
+        (void)ref;
+        return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(b, sy, grid, B, s, dev)
+                            : launch_persistent_t<double, kPhilox, false, PMAX, K>(b, sy, grid, B, s, dev);
+    }
+}
 template <int PMAX>
-bool lds_fits(bool f64, int src, bool ref, int B, int dev) {
-    if (!f64) return lds_fits_t<float, kPhilox, false, PMAX>(B, dev);
-    if (src == kHost)
-        return ref ? lds_fits_t<double, kHost, true, PMAX>(B, dev) : lds_fits_t<double, kHost, false, PMAX>(B, dev);
-    return ref ? lds_fits_t<double, kPhilox, true, PMAX>(B, dev) : lds_fits_t<double, kPhilox, false, PMAX>(B, dev);
+hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid,
+                             int B, hipStream_t s, int dev) {
+    switch (K) {
+        case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, b, sy, grid, B, s, dev);
+        case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, b, sy, grid, B, s, dev);
+        case 5: return launch_persistent_k<PMAX, 5>(f64, src, ref, b, sy, grid, B, s, dev);
+        case 6: return launch_persistent_k<PMAX, 6>(f64, src, ref, b, sy, grid, B, s, dev);
+        default: return launch_persistent_k<PMAX, 3>(f64, src, ref, b, sy, grid, B, s, dev);
+    }
 }
 
+template <int PMAX, int K>
+bool lds_fits_k(bool f64, int src, bool ref, int B, int dev) {
+    if (!f64) return lds_fits_t<float, kPhilox, false, PMAX, K>(B, dev);
+    if constexpr (K == 3) {
+        if (src == kHost)
+            return ref ? lds_fits_t<double, kHost, true, PMAX>(B, dev)
+                       : lds_fits_t<double, kHost, false, PMAX>(B, dev);
+        return ref ? lds_fits_t<double, kPhilox, true, PMAX>(B, dev)
+                   : lds_fits_t<double, kPhilox, false, PMAX>(B, dev);
+    } else {
+        (void)ref;
+        return src == kHost ? lds_fits_t<double, kHost, false, PMAX, K>(B, dev)
+                            : lds_fits_t<double, kPhilox, false, PMAX, K>(B, dev);
+    }
+}
 template <int PMAX>
-int max_coresident(bool f64, int src, bool ref, int B, int cus, int dev) {
-    if (!f64) return max_coresident_t<float, kPhilox, false, PMAX>(B, cus, dev);
-    if (src == kHost)
-        return ref ? max_coresident_t<double, kHost, true, PMAX>(B, cus, dev)
-                   : max_coresident_t<double, kHost, false, PMAX>(B, cus, dev);
-    return ref ? max_coresident_t<double, kPhilox, true, PMAX>(B, cus, dev)
-               : max_coresident_t<double, kPhilox, false, PMAX>(B, cus, dev);
+bool lds_fits(int K, bool f64, int src, bool ref, int B, int dev) {
+    switch (K) {
+        case 2: return lds_fits_k<PMAX, 2>(f64, src, ref, B, dev);
+        case 4: return lds_fits_k<PMAX, 4>(f64, src, ref, B, dev);
+        case 5: return lds_fits_k<PMAX, 5>(f64, src, ref, B, dev);
+        case 6: return lds_fits_k<PMAX, 6>(f64, src, ref, B, dev);
+        default: return lds_fits_k<PMAX, 3>(f64, src, ref, B, dev);
+    }
+}
+
+template <int PMAX, int K>
+int max_coresident_k(bool f64, int src, bool ref, int B, int cus, int dev) {
+    if (!f64) return max_coresident_t<float, kPhilox, false, PMAX, K>(B, cus, dev);
+    if constexpr (K == 3) {
+        if (src == kHost)
+            return ref ? max_coresident_t<double, kHost, true, PMAX>(B, cus, dev)
+                       : max_coresident_t<double, kHost, false, PMAX>(B, cus, dev);
+        return ref ? max_coresident_t<double, kPhilox, true, PMAX>(B, cus, dev)
+                   : max_coresident_t<double, kPhilox, false, PMAX>(B, cus, dev);
+    } else {
+        (void)ref;
+        return src == kHost ? max_coresident_t<double, kHost, false, PMAX, K>(B, cus, dev)
+                            : max_coresident_t<double, kPhilox, false, PMAX, K>(B, cus, dev);
+    }
+}
+template <int PMAX>
+int max_coresident(int K, bool f64, int src, bool ref, int B, int cus, int dev) {
+    switch (K) {
+        case 2: return max_coresident_k<PMAX, 2>(f64, src, ref, B, cus, dev);
+        case 4: return max_coresident_k<PMAX, 4>(f64, src, ref, B, cus, dev);
+        case 5: return max_coresident_k<PMAX, 5>(f64, src, ref, B, cus, dev);
+        case 6: return max_coresident_k<PMAX, 6>(f64, src, ref, B, cus, dev);
+        default: return max_coresident_k<PMAX, 3>(f64, src, ref, B, cus, dev);
+    }
 }
 
 // Persistent LSM kernels run one at a time per device (two of them could each
@@ -1203,10 +1507,10 @@ int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exa
     pcf_opts_init(&o);
     const LsmShape sh = lsm_shape(N, f64 ? 1 : 0, o);
     const int64_t grid_max = std::max(sh.grid_paths, sh.grid_step);
-    // + parameters and sync words (< 8 KB) + the persistent form's partials:
-    // at most 4096 co-resident blocks x 192 B
+    // + coefficients, discounts, parameters and sync words + the persistent
+    // form's partials: at most 4096 co-resident blocks x 448 B (6-term basis)
     return (int64_t)M * sh.cap * rs + sh.cap * (4 + 2 * rs) + grid_max * 10 * 16 +
-           8 * (5 * (int64_t)(M + 1)) + 8192 + 4096 * 192;
+           8 * (9 * (int64_t)(M + 1)) + 8192 + 4096 * 448;
 }
 
 // Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
@@ -1239,6 +1543,17 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const bool ref = o.ref_exact != 0;
     const bool f64 = ref || src == kHost || o.precision == PCF_FP64;
     const size_t rs = f64 ? sizeof(double) : sizeof(float);
+    // Regression basis: K = 3 ({1, S/E, (S/E)^2}; the reference's own basis in
+    // the reference-exact mode) or the K-term basis in x = S/E - 1.
+    const int K = o.lsm_basis == 0 ? 3 : o.lsm_basis;
+    if (K < 2 || K > 6)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "lsm_basis must be 0 (3 terms) or 2..6, got %d", o.lsm_basis);
+    if (K != 3 && ref)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
+                        "the reference-exact mode regresses on the reference's 3-term basis (lsm_basis %d)", K);
+    if (K != 3 && o.kernel_variant == 1)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "a %d-term basis runs in the persistent form only", K);
+    const int CS = K == 3 ? 4 : 8;  // per-date coefficient stride: flag + K coefficients
 
     // Parameter block (host copy -> device), discount table, buffers.
     const double dt = T / M;
@@ -1271,8 +1586,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
     // coef (4 per date) | discount table | parameter block | sync words
     // coef | disc | params | (256-byte aligned) persistent-form sync words
-    const size_t coef_bytes = sizeof(double) * (4 * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) + 512 +
-                              sizeof(unsigned int) * kSyncWords;
+    const size_t coef_bytes = sizeof(double) * (CS * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) +
+                              512 + sizeof(unsigned int) * kSyncWords;
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
@@ -1291,15 +1606,15 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     b.st = sp + (size_t)sh.cap * (sizeof(int32_t) + rs);
     double* cbase = ctx->lsm_coef.as<double>();
     b.coef = cbase;
-    b.disc = cbase + 4 * (size_t)(M + 1);
-    LsmParams* Pdev = (LsmParams*)(cbase + 5 * (size_t)(M + 1));
+    b.disc = cbase + CS * (size_t)(M + 1);
+    LsmParams* Pdev = (LsmParams*)(cbase + (CS + 1) * (size_t)(M + 1));
     b.P = Pdev;
     b.partials = ctx->partials.as<pcf::CSum>();
     b.counter = ctx->counters.as<unsigned int>();
     b.out = ctx->out.as<double>();
 
-    std::vector<double> host_blk(5 * (size_t)(M + 1), 0.0);
-    for (int k = 0; k <= M; ++k) host_blk[4 * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
+    std::vector<double> host_blk((CS + 1) * (size_t)(M + 1), 0.0);
+    for (int k = 0; k <= M; ++k) host_blk[CS * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
 
     if ((st = pcf_timing_begin(ctx))) return st;
     if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
@@ -1326,7 +1641,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     // opts, else lsm_persistent_shape.
     bool persistent = false;
     int pgrid = 0, pmax = 0, PB = 0;
-    if (o.kernel_variant == 2 && M + 1 > kDiscLds)
+    if ((o.kernel_variant == 2 || K != 3) && M + 1 > kDiscLds)
         return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "persistent LSM form holds at most %d dates (M=%d)", kDiscLds - 1,
                         M);
     if (o.kernel_variant != 1 && M + 1 <= kDiscLds) {
@@ -1337,7 +1652,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             if (o.block_size > 0) bs = o.block_size;
             if (o.items_per_thread > 0) pm = o.items_per_thread;
         } else {
-            pcf_tuned_shape(PCF_KIND_AMER, f64 ? 1 : 0, N, &bs, &pm);
+            if (K == 3) pcf_tuned_shape(PCF_KIND_AMER, f64 ? 1 : 0, N, &bs, &pm);  // tuned for the quadratic
             if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, ctx->cu_count, &bs, &pm);
         }
         if (bs % 64 || bs < 64 || bs > 1024 || pm < 1 || pm > 8)
@@ -1348,15 +1663,15 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         while (p2 < pm) p2 <<= 1;
         const int64_t g = (P.half + (int64_t)p2 * bs - 1) / ((int64_t)p2 * bs);
         int gmax;
-        if (p2 == 1) gmax = max_coresident<1>(f64, src, ref, bs, ctx->cu_count, ctx->device);
-        else if (p2 == 2) gmax = max_coresident<2>(f64, src, ref, bs, ctx->cu_count, ctx->device);
-        else if (p2 == 4) gmax = max_coresident<4>(f64, src, ref, bs, ctx->cu_count, ctx->device);
-        else gmax = max_coresident<8>(f64, src, ref, bs, ctx->cu_count, ctx->device);
+        if (p2 == 1) gmax = max_coresident<1>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 2) gmax = max_coresident<2>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 4) gmax = max_coresident<4>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else gmax = max_coresident<8>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         bool lds_ok;
-        if (p2 == 1) lds_ok = lds_fits<1>(f64, src, ref, bs, ctx->device);
-        else if (p2 == 2) lds_ok = lds_fits<2>(f64, src, ref, bs, ctx->device);
-        else if (p2 == 4) lds_ok = lds_fits<4>(f64, src, ref, bs, ctx->device);
-        else lds_ok = lds_fits<8>(f64, src, ref, bs, ctx->device);
+        if (p2 == 1) lds_ok = lds_fits<1>(K, f64, src, ref, bs, ctx->device);
+        else if (p2 == 2) lds_ok = lds_fits<2>(K, f64, src, ref, bs, ctx->device);
+        else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
+        else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
         if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
         if (g >= 1 && g <= gmax) {
             persistent = true;
@@ -1364,7 +1679,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             pmax = p2;
             PB = bs;
         }
-        if (o.kernel_variant == 2 && !persistent)
+        if ((o.kernel_variant == 2 || K != 3) && !persistent)
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                             "persistent LSM form does not fit N=%lld (%lld blocks of %d, at most %d co-resident)",
                             (long long)N, (long long)g, bs, gmax);
@@ -1372,7 +1687,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     // Read-back of the per-date flags and the persistent form's fin/err words,
     // staged in the context's pinned buffer (after out[0..3]) so the one
     // synchronisation in pcf_timing_end covers every copy.
-    const size_t ncoef = 4 * (size_t)(M + 1);
+    const size_t ncoef = CS * (size_t)(M + 1);
     const bool coef_pinned = 8 + ncoef <= (size_t)kOutDoubles;
     unsigned int* const sync_pin = reinterpret_cast<unsigned int*>(ctx->h_out + 4);
     sync_pin[0] = sync_pin[1] = sync_pin[2] = sync_pin[3] = 0u;
@@ -1386,7 +1701,10 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         return PCF_OK;
     };
     if (persistent) {
-        PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid * (2 * 8 * sizeof(double) + 4 * sizeof(pcf::CSum))));
+        // 2 date parities x 8C sums, then the final sums' 4 compensated pairs
+        const size_t chunks = K == 6 ? 3 : (K >= 4 ? 2 : 1);
+        PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid *
+                                                (2 * 8 * chunks * sizeof(double) + 4 * sizeof(pcf::CSum))));
         b.partials = ctx->partials.as<pcf::CSum>();
         const int fin_w = kArrivalCounters * kCounterStride;
         LsmSync sy{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
@@ -1401,10 +1719,10 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, cs));
             hipError_t le;
-            if (pmax == 1) le = launch_persistent<1>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else if (pmax == 2) le = launch_persistent<2>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else if (pmax == 4) le = launch_persistent<4>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else le = launch_persistent<8>(f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            else le = launch_persistent<8>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, cs));
         }
@@ -1484,7 +1802,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
     int small = 0, fallback = 0, singular = 0;
     for (int m = 1; m <= M - 1; ++m) {
-        const int f = (int)cf_host[4 * m];
+        const int f = (int)cf_host[CS * m];
         small += f == kFlagSmall;
         fallback += f == kFlagReduced;
         singular += f == kFlagSingular;

@@ -3,6 +3,8 @@
 // Default: corrected Longstaff-Schwartz (exercise test on payoff(S)).
 // --ref-exact (implied by --rng mt19937) reproduces the reference's arithmetic,
 // including its payoff(S-E,E) exercise test (src/mc_amer.cpp:100).
+// --basis K (2..6) regresses on {1, x, .., x^(K-1)}, x = S/E - 1, instead of the
+// reference's three terms (production arithmetic only).
 #include "cli_common.hpp"
 
 int main(int argc, char** argv) {
@@ -10,6 +12,7 @@ int main(int argc, char** argv) {
     cli::Args a = cli::parse(argc, argv, 8, "mc_amer call|put S0 E r sigma T N M [flags]");
     pcf_ctx* ctx = cli::open_ctx();
     pcf_opts o = cli::base_opts(a);
+    if (const char* s = cli::extra(a, "--basis")) o.lsm_basis = (int32_t)cli::inum(s, "--basis");
     std::vector<double> w;
     if (a.mt19937) {
         if (a.N <= 0 || a.M <= 0 || a.N % 2) cli::die("N needs to be divisible by 2 for finding paths");

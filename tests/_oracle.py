@@ -25,6 +25,9 @@ _SIG = {
     "orc_inverse3_solve": (_i32, [_pd, _pd, _pd]),
     "orc_mc_amer_z": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd, _i32, C.POINTER(C.c_int32)]),
     "orc_mc_amer_seed": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _u64, _i32]),
+    "orc_lsm_solve_poly": (_i32, [_pd, _i32, _pd]),
+    "orc_mc_amer_poly": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd, _i32, C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int32)]),
     "orc_basket_z": (_d, [_d, _d, _d, _d, _d, _i64, _d, _i32, _pd, _pd, _pd, _pd, _pd]),
     "orc_cholesky": (_i32, [_i32, _pd, _pd]),
     "orc_philox4x32_10": (None, [C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),

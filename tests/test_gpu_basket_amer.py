@@ -256,3 +256,80 @@ def test_amer_persistent_matches_graph_form(engine, N, M, prec, ref):
     else:
         assert abs(g.price - p.price) <= 0.01 * g.std_err
     assert g.control_price == p.control_price or abs(g.control_price - p.control_price) < 1e-12 * g.control_price
+
+
+# ---- K-term regression basis {1, x, .., x^(K-1)}, x = S/E - 1 -------------------
+
+def _amer_poly_ref(N, M, pf, w, K):
+    when = np.zeros(N, dtype=np.int32)
+    flags = np.zeros(M + 1, dtype=np.int32)
+    I32 = O.C.POINTER(O.C.c_int32)
+    v = O.lib().orc_mc_amer_poly(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), K, when.ctypes.data_as(I32),
+                                 flags.ctypes.data_as(I32))
+    return v, when, int((flags[1:M] == 4).sum())
+
+
+@pytest.mark.parametrize("K,pf", [(2, "put"), (4, "put"), (5, "call"), (6, "put")])
+def test_amer_basis_host_normals_vs_oracle(engine, K, pf):
+    """FP64 on the host's normals: the device's K-term regression makes the
+    oracle's decisions (up to a flip at the rounding boundary of a fit, from
+    the different summation order) and its price."""
+    N, M = 8192, 50
+    w = O.normals_mt(70 + K, math.sqrt(1.0 / M), (N // 2) * M)
+    p = 1.0 if pf == "call" else -1.0
+    want, wwhen, wred = _amer_poly_ref(N, M, p, w, K)
+    say("amer basis host normals", K, pf)
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, pf, normals=w, lsm_basis=K, return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    say("  flips", flips, "reduced dates", got.lsm_fallback, wred)
+    assert flips <= 2
+    assert rel(got.price, want) < (1e-11 if flips == 0 else 1e-4)
+    assert got.lsm_fallback == wred
+
+
+@pytest.mark.parametrize("K", [2, 4, 6])
+def test_amer_basis_fp64_philox_vs_oracle(engine, K):
+    N, M = 16384, 30
+    w = np.empty((N // 2) * M)
+    O.lib().orc_philox_amer_normals(17, N, M, 1.0, O.ptr(w))
+    want, wwhen, _ = _amer_poly_ref(N, M, -1.0, w, K)
+    say("amer basis fp64 philox", K)
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=17, precision="fp64", lsm_basis=K,
+                         return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    assert flips <= 2
+    assert rel(got.price, want) < (1e-11 if flips == 0 else 1e-4)
+
+
+@pytest.mark.parametrize("K", [4, 5])
+def test_amer_basis_fp32_at_full_size(engine, K):
+    """2^20 paths, 50 dates (the American workload's size): FP32 paths within
+    the standard error of FP64, both above the European put and near the
+    reference lattice (binom_vanilla_amer N=2000: 5.7989)."""
+    N, M = 1 << 20, 50
+    say("amer basis fp32 full size", K)
+    f32 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=3, lsm_basis=K)
+    f64 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=3, precision="fp64", lsm_basis=K)
+    q = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=3)
+    say("  K", K, f32.price, f64.price, "quadratic", q.price, "se", f64.std_err, "kernel_ms", f32.kernel_ms)
+    assert abs(f32.price - f64.price) < f64.std_err
+    eur = O.lib().orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)
+    assert eur < f32.price and abs(f32.price - 5.798877611783797) < 0.03
+    assert abs(f32.price - q.price) < 0.02  # more terms move the estimate by little
+
+
+def test_amer_basis_errors(engine):
+    from parcompfin_amd import PcfError
+    say("amer basis errors")
+    for kw in (dict(lsm_basis=7), dict(lsm_basis=1), dict(lsm_basis=-2), dict(lsm_basis=4, ref_exact=True),
+               dict(lsm_basis=4, kernel_variant=1)):
+        with pytest.raises(PcfError):
+            engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", **kw)
+    with pytest.raises(PcfError):  # more dates than the persistent form holds
+        engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 1100, "put", lsm_basis=4)
+    # 0 and 3 are the same quadratic; M = 1 has no regression date
+    a = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", seed=4)
+    b = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", seed=4, lsm_basis=3)
+    assert a.price == b.price
+    r = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 1, "put", seed=4, lsm_basis=5)
+    assert abs(r.price - r.control_price) < 1e-12 * r.price  # no date to exercise on: the European value

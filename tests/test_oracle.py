@@ -249,6 +249,142 @@ def test_lsm_against_numpy_restatement(pf, quirk):
     assert abs(got - want) < 1e-10 * abs(want)
 
 
+def _poly_sums(x, y, K):
+    g = np.zeros(3 * K - 1)
+    for j in range(2 * K - 1):
+        g[j] = (x ** j).sum()
+    for i in range(K):
+        g[2 * K - 1 + i] = (y * x ** i).sum()
+    return g
+
+
+def _solve_poly(g, K):
+    c = np.zeros(K + 1)
+    flag = O.lib().orc_lsm_solve_poly(O.ptr(np.ascontiguousarray(g)), K, O.ptr(c))
+    assert flag == int(c[0])
+    return flag, c[1:]
+
+
+@pytest.mark.parametrize("K", [2, 3, 4, 5, 6])
+def test_lsm_poly_solve_against_numpy(K):
+    """The K x K Cholesky solve of the Hankel normal equations: numpy's solve of
+    the same system, numpy's least squares on the design matrix, and exact
+    recovery of a polynomial of degree K-1."""
+    rng = np.random.default_rng(K)
+    x = rng.uniform(0.55, 1.0, 5000)  # S/E of in-the-money puts
+    y = np.maximum(1.0 - x, 0) * 0.97 + 0.02 * rng.standard_normal(x.size)
+    g = _poly_sums(x, y, K)
+    flag, c = _solve_poly(g, K)
+    assert flag == 2
+    H = np.array([[g[i + j] for j in range(K)] for i in range(K)])
+    want = np.linalg.solve(H, g[2 * K - 1:])
+    cond = np.linalg.cond(H)
+    fit, want_fit = np.polyval(c[::-1], x), np.polyval(want[::-1], x)
+    assert np.max(np.abs(fit - want_fit)) < 1e-15 * cond
+    X = np.vander(x, K, increasing=True)
+    ls = np.linalg.lstsq(X, y, rcond=None)[0]
+    assert np.max(np.abs(fit - X @ ls)) < 1e-15 * cond
+    # y exactly a polynomial of degree K-1 in x: the fit reproduces it.
+    p = rng.uniform(-1, 1, K)
+    yp = np.polyval(p[::-1], x)
+    flag, c = _solve_poly(_poly_sums(x, yp, K), K)
+    assert flag == 2
+    assert np.max(np.abs(np.polyval(c[::-1], x) - yp)) < 1e-15 * cond * np.abs(yp).max()
+
+
+def test_lsm_poly_solve_special_cases():
+    K = 4
+    assert _solve_poly(np.zeros(3 * K - 1), K)[0] == 0  # nothing in the money
+    x = np.array([0.8, 0.9])
+    assert _solve_poly(_poly_sums(x, np.array([0.1, 0.2]), K), K)[0] == 1  # two paths
+    # One distinct x value: only the constant term survives, c0 = mean(y).
+    x = np.full(7, 0.9)
+    y = np.arange(7.0)
+    flag, c = _solve_poly(_poly_sums(x, y, K), K)
+    assert flag == 4 and abs(c[0] - y.mean()) < 1e-12 and not c[1:].any()
+    # Two distinct values: a straight line through both means, higher terms 0.
+    x = np.array([0.7, 0.7, 0.7, 0.9, 0.9])
+    y = np.array([1.0, 2.0, 3.0, 5.0, 7.0])
+    flag, c = _solve_poly(_poly_sums(x, y, K), K)
+    assert flag == 4 and not c[2:].any()
+    assert abs(c[0] + c[1] * 0.7 - 2.0) < 1e-9 and abs(c[0] + c[1] * 0.9 - 6.0) < 1e-9
+
+
+def _lsm_poly_numpy(S0, E, r, sigma, T, N, M, pf, w, K):
+    """Independent LSM with the K-term basis {1, x, ..., x^(K-1)}, x = S/E - 1: numpy
+    least squares on the design matrix (no normal equations)."""
+    dt = T / M
+    incr = (r - 0.5 * sigma * sigma) * dt
+    P = np.empty((M + 1, N))
+    P[0] = S0
+    W = w.reshape(N // 2, M)
+    for m in range(1, M + 1):
+        P[m, :N // 2] = P[m - 1, :N // 2] * np.exp(incr + sigma * W[:, m - 1])
+        P[m, N // 2:] = P[m - 1, N // 2:] * np.exp(incr - sigma * W[:, m - 1])
+    pay = lambda S: np.maximum(pf * (S - E), 0.0)
+    disc = np.exp(-r * dt * np.arange(M + 1))
+    when = np.full(N, M)
+    cf = pay(P[M])
+    for m in range(M - 1, 0, -1):
+        pv = pay(P[m])
+        itm = np.nonzero(pv > 0)[0]
+        if itm.size == 0:
+            continue
+        y = disc[when[itm] - m] * cf[itm]
+        if itm.size <= 2:
+            v = y
+        else:
+            X = np.vander(P[m, itm] / E - 1.0, K, increasing=True)
+            v = X @ np.linalg.lstsq(X, y, rcond=None)[0]
+        ex = itm[pv[itm] > v]
+        when[ex] = m
+        cf[ex] = pv[ex]
+    return max(pay(np.array(S0)).item(), (disc[when] * cf).sum() / N), when
+
+
+@pytest.mark.parametrize("K,pf", [(2, -1.0), (4, -1.0), (5, 1.0), (6, -1.0)])
+def test_lsm_poly_against_numpy_least_squares(K, pf):
+    N, M = 2048, 12
+    w = O.normals_mt(57 + K, math.sqrt(1.0 / M), (N // 2) * M)
+    when = np.zeros(N, dtype=np.int32)
+    flags = np.zeros(M + 1, dtype=np.int32)
+    I32 = O.C.POINTER(O.C.c_int32)
+    got = O.lib().orc_mc_amer_poly(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), K,
+                                   when.ctypes.data_as(I32), flags.ctypes.data_as(I32))
+    want, wwhen = _lsm_poly_numpy(100, 100, 0.06, 0.2, 1.0, N, M, pf, w, K)
+    assert set(flags[1:M]) == {2}  # full basis at every date, as lstsq assumes
+    assert (when != wwhen).sum() <= 1  # a decision within rounding of its fit may flip
+    assert abs(got - want) < (1e-11 if (when == wwhen).all() else 1e-4) * want
+
+
+def test_lsm_poly_k3_is_the_quadratic_regression():
+    """K = 3 on S/E fits the same quadratic as the reference basis {1, S-E,
+    (S-E)^2} (orc_mc_amer_z, corrected exercise test): same decisions, same
+    price up to rounding."""
+    N, M = 8192, 50
+    w = O.normals_mt(5, math.sqrt(1.0 / M), (N // 2) * M)
+    I32 = O.C.POINTER(O.C.c_int32)
+    for pf in (-1.0, 1.0):
+        wa, wb = np.zeros(N, dtype=np.int32), np.zeros(N, dtype=np.int32)
+        a = O.lib().orc_mc_amer_poly(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), 3, wa.ctypes.data_as(I32), None)
+        b = O.lib().orc_mc_amer_z(100, 100, 0.06, 0.2, 1.0, N, M, pf, O.ptr(w), 0, wb.ctypes.data_as(I32))
+        assert (wa != wb).sum() <= 2
+        assert abs(a - b) < (1e-11 if (wa == wb).all() else 1e-4) * b
+
+
+def test_lsm_poly_prices_bracket():
+    """Every basis size prices the put above the European put and near the
+    reference lattice (binom_vanilla_amer, N=2000)."""
+    L = O.lib()
+    N, M = 1 << 14, 50
+    w = np.empty((N // 2) * M)
+    L.orc_philox_amer_normals(11, N, M, 1.0, O.ptr(w))
+    eur = L.orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)
+    for K in (2, 3, 4, 5, 6):
+        v = L.orc_mc_amer_poly(100, 100, 0.06, 0.2, 1.0, N, M, -1.0, O.ptr(w), K, None, None)
+        assert eur < v and abs(v - 5.798877611783797) < 0.1, (K, v)
+
+
 def test_lsm_production_put_brackets():
     """Corrected LSM put: above the European put, near the reference lattice."""
     L = O.lib()
