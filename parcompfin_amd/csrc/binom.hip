@@ -12,8 +12,11 @@
 //  * chain (reference-exact): log C(N,i) replayed bit for bit as the
 //    reference's sequential chain (include/common.h:63-72):
 //        acc = sum_{n=N..i+1} log n  (descending),  then acc -= log j, j=2..N-i
-//    with glibc log values from a host table.  The table index is uniform
-//    across a wavefront, so loads are scalar broadcasts; lanes that have
+//    with glibc log values from a host table.  The descending part of every
+//    term is a prefix of one chain (log N + log(N-1) + ... in that order), so
+//    it comes from a host-made prefix table, rounding for rounding; only the
+//    second part runs per lane.  Its table index is uniform across a
+//    wavefront, so loads are scalar broadcasts (32 per wait); lanes that have
 //    reached their own end are masked, never reordered.
 //  * lgamma (production): lgamma(N+1) - lgamma(i+1) - lgamma(N-i+1) in FP64.
 #include <cmath>
@@ -43,6 +46,7 @@ __device__ __forceinline__ double term_value(const BinomArgs& a, double comb_val
 // Chain form.  One lane per term index; all 64 lanes of a wave run the same
 // trip counts (wave-uniform loop bounds from the wave's first index).
 __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* __restrict__ tab,
+                                                     const double* __restrict__ pre,
                                                      pcf::CSum* __restrict__ partials,
                                                      unsigned int* __restrict__ counter,
                                                      double* __restrict__ out) {
@@ -52,40 +56,20 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(pcf::kWave - 1));
     const int N = (int)a.N;
     const int w0 = __builtin_amdgcn_readfirstlane((int)i0);
-    double acc = 0.0;
-    // Part 1: n = N down to i+1.  Unmasked while n > i0+63 (every lane has i <= i0+63).
-    {
-        int n = N;
-        const int lo = w0 + pcf::kWave;  // unmasked for n >= lo
-        for (; n - 7 >= lo; n -= 8) {
-            acc += tab[n];
-            acc += tab[n - 1];
-            acc += tab[n - 2];
-            acc += tab[n - 3];
-            acc += tab[n - 4];
-            acc += tab[n - 5];
-            acc += tab[n - 6];
-            acc += tab[n - 7];
-        }
-        for (; n >= lo; --n) acc += tab[n];
-        for (; n > w0; --n) {
-            const double t = tab[n];
-            acc = (n > i) ? acc + t : acc;
-        }
-    }
+    // Part 1: n = N down to i+1, i.e. the first N-i steps of the descending
+    // chain (pre[k], k = 0..N; lanes past N/2 + 1 carry no term).
+    const int64_t k1 = a.N - i;
+    double acc = pre[k1 < 0 ? 0 : (k1 > a.N ? a.N : k1)];
     // Part 2: j = 2 .. N-i.  Unmasked while j <= N-(i0+63).
     {
         int j = 2;
         const int hi_all = N - (w0 + pcf::kWave - 1);
-        for (; j + 7 <= hi_all; j += 8) {
-            acc -= tab[j];
-            acc -= tab[j + 1];
-            acc -= tab[j + 2];
-            acc -= tab[j + 3];
-            acc -= tab[j + 4];
-            acc -= tab[j + 5];
-            acc -= tab[j + 6];
-            acc -= tab[j + 7];
+        for (; j + 31 <= hi_all; j += 32) {
+            double c[32];  // all 32 loads issued before the first subtraction waits
+#pragma unroll
+            for (int t = 0; t < 32; ++t) c[t] = tab[j + t];
+#pragma unroll
+            for (int t = 0; t < 32; ++t) acc -= c[t];
         }
         for (; j <= hi_all; ++j) acc -= tab[j];
         const int hi_any = N - w0;
@@ -126,16 +110,28 @@ __global__ __launch_bounds__(256) void k_binom_lgamma(BinomArgs a, pcf::CSum* __
         out[0] = tot.v[0].s + tot.v[0].c;
 }
 
-// glibc log(n) for n = 0..N (entry 0 unused), resident per N.
+// glibc log(n) for n = 0..N (entry 0 unused) and the descending chain's
+// prefixes pre[k] = (..((0 + log N) + log(N-1)) + ..) + log(N-k+1), k = 0..N,
+// made in that order on the host (plain IEEE double adds), resident per N.
 int ensure_logtab(pcf_ctx* ctx, int64_t N) {
-    if (ctx->logtab_n == N) return PCF_OK;
-    std::vector<double> t((size_t)N + 1);
+    if (ctx->logtab_n == N && ctx->prefix_n == N) return PCF_OK;
+    std::vector<double> t((size_t)N + 1), pre((size_t)N + 1);
     t[0] = 0.0;
     for (int64_t n = 1; n <= N; ++n) t[(size_t)n] = std::log((double)n);
+    volatile double acc = 0.0;  // one rounding per add, in the reference's order
+    pre[0] = 0.0;
+    for (int64_t k = 1; k <= N; ++k) {
+        acc = acc + t[(size_t)(N - k + 1)];
+        pre[(size_t)k] = acc;
+    }
     PCF_HIP_CHECK(ctx, ctx->logtab.ensure(t.size() * sizeof(double)));
+    PCF_HIP_CHECK(ctx, ctx->prefix.ensure(pre.size() * sizeof(double)));
     PCF_HIP_CHECK(ctx, hipMemcpy(ctx->logtab.ptr, t.data(), t.size() * sizeof(double),
                                  hipMemcpyHostToDevice));
+    PCF_HIP_CHECK(ctx, hipMemcpy(ctx->prefix.ptr, pre.data(), pre.size() * sizeof(double),
+                                 hipMemcpyHostToDevice));
     ctx->logtab_n = N;
+    ctx->prefix_n = N;
     return PCF_OK;
 }
 
@@ -183,7 +179,7 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
-        k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(),
+        k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
                                                   ctx->partials.as<pcf::CSum>(),
                                                   ctx->counters.as<unsigned int>(),
                                                   ctx->out.as<double>());
