@@ -61,7 +61,7 @@ struct LsmBufs {
     int32_t* when;
     void* cf;
     void* st;
-    double* coef;  // 4 per date, dates 0..M
+    double* coef;  // per date 0..M: flag + coefficients (stride 4 for the 3-term basis, else 8)
     double* disc;  // exp(-r dt k), k = 0..M
     const double* hw;  // host normals (verification)
     pcf::CSum* partials;

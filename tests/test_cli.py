@@ -113,6 +113,20 @@ def test_comparison_column_from_the_environment():
 
 
 @pytest.mark.gpu
+def test_amer_basis_flag():
+    # --basis K: the K-term regression; with --stats the fallback count is printed.
+    p = run("mc_amer", "put", 100, 100, 0.06, 0.2, 1, 1 << 16, 50, "--seed", 5, "--basis", 4, "--stats")
+    assert p.returncode == 0, p.stderr
+    got = parse_row(p.stdout)
+    assert got["Method"] == "HIP" and abs(float(got["Result"]) - 5.7989) < 0.05
+    q = run("mc_amer", "put", 100, 100, 0.06, 0.2, 1, 1 << 16, 50, "--seed", 5)
+    assert parse_row(q.stdout)["Result"] != got["Result"]  # a different regression
+    for bad in (["--basis", 9], ["--basis", 4, "--ref-exact"], ["--basis", 4, "--rng", "mt19937", "--seed", 1]):
+        p = run("mc_amer", "put", 100, 100, 0.06, 0.2, 1, 4096, 50, *bad)
+        assert p.returncode == 1 and "basis" in p.stderr, (bad, p.stderr)
+
+
+@pytest.mark.gpu
 def test_trailing_thread_count_is_accepted():
     # the _omp programs' argv (src/mc_eur_omp.cpp:41): a trailing thread count
     p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 100000, 16)
