@@ -721,29 +721,67 @@ __device__ __forceinline__ void block_sum_plain(double v[K], double* lds) {
 // every lane whose bits 3..5 encode k (k = b3 + 2 b4 + 4 b5) holds sum k.
 // 10 FP64 exchanges instead of 48; IEEE addition commutes, so both partners
 // of an exchange form the same sum and the order is fixed.
+// Lane exchanges without the LDS crossbar (gfx950): DPP within 16-lane rows
+// (row_ror:n gives lane l the value of lane (l - n) mod 16; quad_perm 0x4E and
+// 0xB1 are xor 2 and xor 1), and v_permlane32_swap / v_permlane16_swap, which
+// swap the upper 32 lanes of x with the lower 32 of y (and the odd 16-lane
+// rows of x with the even rows of y).  FP64 travels as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <bool ROWS16>
+__device__ __forceinline__ void permlane_swap_f64(double& x, double& y) {
+    const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+    const uint32_t xl = (uint32_t)ux, xh = (uint32_t)(ux >> 32), yl = (uint32_t)uy, yh = (uint32_t)(uy >> 32);
+    uint32_t rl0, rl1, rh0, rh1;
+    if constexpr (ROWS16) {
+        const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        rl0 = l[0]; rl1 = l[1]; rh0 = h[0]; rh1 = h[1];
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        rl0 = l[0]; rl1 = l[1]; rh0 = h[0]; rh1 = h[1];
+    }
+    x = __builtin_bit_cast(double, ((uint64_t)rh0 << 32) | rl0);
+    y = __builtin_bit_cast(double, ((uint64_t)rh1 << 32) | rl1);
+}
+
+// (The exchanges below pair exactly the lanes of the xor-32/16/8/4/2/1
+// shuffles described above and add the same two operands, so the sums are
+// the shuffles' sums bit for bit: after a swap, lane l holds its own kept
+// value and its partner's given value in x and y, and x + y == y + x.)
 __device__ __forceinline__ double wave_sum8(const double v[8]) {
     const int lane = threadIdx.x & (pcf::kWave - 1);
-    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
     double a[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const double keep = b5 ? v[j + 4] : v[j];
-        const double give = b5 ? v[j] : v[j + 4];
-        a[j] = keep + __shfl_xor(give, 32, pcf::kWave);
+    for (int j = 0; j < 4; ++j) {  // xor 32: lanes < 32 keep v[j], lanes >= 32 keep v[j + 4]
+        double x = v[j], y = v[j + 4];
+        permlane_swap_f64<false>(x, y);
+        a[j] = x + y;
     }
     double c[2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const double keep = b4 ? a[j + 2] : a[j];
-        const double give = b4 ? a[j] : a[j + 2];
-        c[j] = keep + __shfl_xor(give, 16, pcf::kWave);
+    for (int j = 0; j < 2; ++j) {  // xor 16: even rows keep a[j], odd rows keep a[j + 2]
+        double x = a[j], y = a[j + 2];
+        permlane_swap_f64<true>(x, y);
+        c[j] = x + y;
     }
+    const bool b3 = lane & 8, b2 = lane & 4;
     const double keep = b3 ? c[1] : c[0];
     const double give = b3 ? c[0] : c[1];
-    double d = keep + __shfl_xor(give, 8, pcf::kWave);
-    d += __shfl_xor(d, 4, pcf::kWave);
-    d += __shfl_xor(d, 2, pcf::kWave);
-    d += __shfl_xor(d, 1, pcf::kWave);
+    double d = keep + dpp_f64<0x128>(give);                      // xor 8: row_ror:8
+    // xor 4: row_ror:4 for lanes with bit 2 set, row_ror:12 for the others.  Both
+    // run in every lane before the select (a DPP in a divergent branch would
+    // read inactive source lanes as 0).
+    const double r4 = dpp_f64<0x124>(d), r12 = dpp_f64<0x12C>(d);
+    d += b2 ? r4 : r12;
+    d += dpp_f64<0x4E>(d);                                       // xor 2
+    d += dpp_f64<0xB1>(d);                                       // xor 1
     return d;
 }
 
