@@ -53,12 +53,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+METRIC = "paths·steps/s FP32-Kahan vs FP64 at ≤1 SE; p50/p99 latency of served mix"
+
+
 def metric_name():
-    try:
-        with open(os.path.join(ROOT, "BASELINE.json")) as f:
-            return json.load(f)["metric"]
-    except Exception:
-        return "paths·steps/s FP32-Kahan vs FP64 at ≤1 SE; p50/p99 latency of served mix"
+    return METRIC
 
 
 def cpu_model():
@@ -170,30 +169,30 @@ def per_config(eng):
 
     bs = 10.450583572185565
     r32, r64 = pair(lambda p: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=1, precision=p))
-    summary("cfg0_mc_eur_1e6", r32, r64, 1, {"black_scholes": bs, "fp64_minus_bs_in_se": (r64.price - bs) / r64.std_err})
+    summary("mc_eur_1e6", r32, r64, 1, {"black_scholes": bs, "fp64_minus_bs_in_se": (r64.price - bs) / r64.std_err})
     for N in (10**5, 10**6):
         eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
         rc = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
         rl = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
-        out[f"cfg1_binom_embar_N{N}"] = {"price_chain": rc.price, "kernel_ms_chain": rc.kernel_ms,
+        out[f"binom_embar_N{N}"] = {"price_chain": rc.price, "kernel_ms_chain": rc.kernel_ms,
                                          "price_lgamma": rl.price, "kernel_ms_lgamma": rl.kernel_ms,
                                          "lgamma_minus_chain_rel": (rl.price - rc.price) / rc.price,
                                          "log_adds_per_s_chain": rc.flops / (rc.kernel_ms * 1e-3)}
-        log(f"cfg1 N={N}", out[f"cfg1_binom_embar_N{N}"])
+        log(f"binom_embar N={N}", out[f"binom_embar_N{N}"])
     for d in (8, 32):
         sig = np.random.default_rng(d).uniform(0.15, 0.35, size=d)
         r32, r64 = pair(lambda p: eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3,
                                                    sigmas=sig, seed=2, precision=p))
-        summary(f"cfg2_basket_d{d}_2^24", r32, r64, 1, {"tflops_fp32k": r32.flops / (r32.kernel_ms * 1e-3) / 1e12})
+        summary(f"basket_d{d}_2^24", r32, r64, 1, {"tflops_fp32k": r32.flops / (r32.kernel_ms * 1e-3) / 1e12})
     r32, r64 = pair(lambda p: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=3, precision=p,
                                           control=True))
     geo = 6.551716578
-    summary("cfg3_mc_asia_2^22x252", r32, r64, 252,
+    summary("mc_asia_2^22x252", r32, r64, 252,
             {"geometric_control_fp32k": r32.control_price, "geometric_closed_form": geo,
              "control_minus_closed_in_se": (r32.control_price - geo) / r32.control_se})
     lat = eng.binom_vanilla(100, 100, 0.06, 0.2, 1.0, 2000, "put", american=True)
     r32, r64 = pair(lambda p: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=4, precision=p))
-    summary("cfg4_mc_amer_put_2^20x50", r32, r64, 50,
+    summary("mc_amer_put_2^20x50", r32, r64, 50,
             {"european_put": 5.166002511, "european_control_fp32k": r32.control_price,
              "lattice_binom_vanilla_amer_N2000": lat.price,
              "fp64_minus_lattice_in_se": (r64.price - lat.price) / r64.std_err,
@@ -281,12 +280,12 @@ def profile_one(eng, which, K, W):
     sig8 = np.random.default_rng(8).uniform(0.15, 0.35, size=8)
     sig32 = np.random.default_rng(32).uniform(0.15, 0.35, size=32)
     calls = {
-        "cfg0": lambda p, s: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=s, precision=p),
-        "cfg1": lambda p, s: eng.binom_embar(100, 100, 0.05, 0.2, 1.0, 10**5, "call", ref_exact=(p == "fp64")),
-        "cfg2": lambda p, s: [eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3, sigmas=sg,
+        "eur": lambda p, s: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=s, precision=p),
+        "binom": lambda p, s: eng.binom_embar(100, 100, 0.05, 0.2, 1.0, 10**5, "call", ref_exact=(p == "fp64")),
+        "basket": lambda p, s: [eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3, sigmas=sg,
                                                seed=s, precision=p) for d, sg in ((8, sig8), (32, sig32))][-1],
-        "cfg3": lambda p, s: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=s, precision=p),
-        "cfg4": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=s, precision=p),
+        "asia": lambda p, s: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=s, precision=p),
+        "amer": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=s, precision=p),
     }
     out = {"profile": which}
     if which == "mix":
@@ -357,13 +356,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--profile", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "mix"],
+    ap.add_argument("--profile", choices=["eur", "binom", "basket", "asia", "amer", "mix"],
                     help="run only one configuration --steps times in FP32/Kahan then FP64 "
                          "(for rocprofv3 kernel traces), print its timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: every box here has one GPU and no multi-GPU path runs in this "
-            "build (how the path would shard is described in DESIGN.md). Use --gpus 1.")
+            "build. Use --gpus 1.")
         sys.exit(2)
     K, W = args.steps, args.warmup
 
@@ -457,8 +456,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded request mix; Philox4x32-10 normals generated on the device)",
-        "config": {"workload": "served mix: European (cfg0), basket d=8 (cfg2), Asian M=252 (cfg3), American "
-                               "put M=50 (cfg4), paths log-uniform in 2^14..2^22, FP32 paths + Kahan/FP64 sums",
+        "config": {"workload": "served mix: European call, basket call d=8, Asian call M=252, American "
+                               "put M=50, paths log-uniform in 2^14..2^22, FP32 paths + Kahan/FP64 sums",
                    "requests_per_step": len(mix), "paths_steps_per_step": ps, "slots": args.slots,
                    "parallelism": "single GPU"},
         "roofline": roof,

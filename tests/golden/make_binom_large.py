@@ -1,4 +1,4 @@
-"""Writes binom_large.csv: the closed binomial sum at N = 1e5 and 1e6 (cfg1)
+"""Writes binom_large.csv: the closed binomial sum at N = 1e5 and 1e6 
 from the oracle's prefix form (oracle/oracle.cpp orc_binom_embar_prefix),
 which equals the reference restatement bit for bit (checked in
 tests/test_oracle.py) and matches the reference's own N=1e5 result in

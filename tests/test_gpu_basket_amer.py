@@ -63,7 +63,7 @@ def test_basket_verification_vs_oracle(engine, d, rho, pf):
 
 
 def test_basket_extension_verification_vs_oracle(engine):
-    """cfg2's extension: per-asset sigma in [0.15, 0.35] and a random SPD correlation."""
+    """The basket extension: per-asset sigma in [0.15, 0.35] and a random SPD correlation."""
     d, N = 8, 20000
     rng = np.random.default_rng(2)
     sig = rng.uniform(0.15, 0.35, size=d)
@@ -98,14 +98,14 @@ def test_basket_fp32_vs_fp64(engine, d, variant):
     assert abs(r32.price - r64.price) < 0.05 * r64.std_err
 
 
-def test_basket_cfg2_full_size(engine):
-    """cfg2: d = 8 and 32, 2^24 paths, sigma_i ~ U[0.15,0.35], rho = 0.3 and a random SPD matrix."""
+def test_basket_2p24_paths_full_size(engine):
+    """d = 8 and 32, 2^24 paths, sigma_i ~ U[0.15,0.35], rho = 0.3 and a random SPD matrix."""
     N = 1 << 24
     for d in (8, 32):
         rng = np.random.default_rng(d)
         sig = rng.uniform(0.15, 0.35, size=d)
         for corr in (equicorr(d, 0.3), random_corr(d, 11)):
-            say("basket cfg2", d)
+            say("basket 2^24 paths", d)
             r64 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, sigmas=sig, corr=corr,
                                       seed=9, precision="fp64")
             r32 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, sigmas=sig, corr=corr, seed=9)
@@ -198,9 +198,9 @@ def test_amer_fp64_philox_vs_oracle(engine):
     assert rel(got.price, want) < (1e-11 if flips == 0 else 1e-4)
 
 
-def test_amer_cfg4_fp32_vs_fp64_and_bounds(engine):
+def test_amer_2p20x50_fp32_vs_fp64_and_bounds(engine):
     N, M = 1 << 20, 50
-    say("amer cfg4 fp32/fp64")
+    say("amer 2^20 x 50 fp32/fp64")
     r64 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=17, precision="fp64")
     r32 = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=17)
     euro = O.lib().orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)

@@ -87,11 +87,11 @@ def test_eur_fp64_philox_vs_oracle(engine):
     assert rel(got.std_err, se.value) < 1e-9
 
 
-def test_eur_cfg0_fp32_vs_fp64_and_black_scholes(engine):
+def test_eur_1e6_paths_fp32_vs_fp64_and_black_scholes(engine):
     N = 10**6
     bs = O.lib().orc_bs_price(100, 100, 0.05, 0.2, 1.0, 1.0)
     for seed in (1, 2, 3):
-        say("mc_eur cfg0 fp32k vs fp64 seed", seed)
+        say("mc_eur 1e6 paths fp32k vs fp64 seed", seed)
         r64 = engine.mc_eur(100, 100, 0.05, 0.2, 1.0, N, "call", seed=seed, precision="fp64")
         r32 = engine.mc_eur(100, 100, 0.05, 0.2, 1.0, N, "call", seed=seed, precision="fp32k")
         assert abs(r32.price - r64.price) < 0.05 * r64.std_err
@@ -154,9 +154,9 @@ def test_asia_fp64_philox_vs_oracle(engine, M):
     assert rel(got.control_price, geo.value) < 1e-11
 
 
-def test_asia_cfg3_fp32_vs_fp64_and_geometric_control(engine):
+def test_asia_2p22_paths_fp32_vs_fp64_and_geometric_control(engine):
     N, M = 1 << 22, 252
-    say("mc_asia cfg3 full size fp32k/fp64 with control")
+    say("mc_asia 2^22 x 252 fp32k/fp64 with control")
     r64 = engine.mc_asia(100, 100, 0.05, 0.25, 1.0, N, M, "call", seed=3, precision="fp64", control=True)
     r32 = engine.mc_asia(100, 100, 0.05, 0.25, 1.0, N, M, "call", seed=3, precision="fp32k", control=True)
     geo = O.lib().orc_geo_asian(100, 100, 0.05, 0.25, 1.0, M, 1.0)
