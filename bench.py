@@ -361,8 +361,8 @@ def main():
                          "(for rocprofv3 kernel traces), print its timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one GPU: every box here has one GPU and no multi-GPU path runs in this "
-            "build. Use --gpus 1.")
+        log("bench.py measures one GPU; the multi-GPU sharding of the path is a design only, with no "
+            "collective code. Use --gpus 1.")
         sys.exit(2)
     K, W = args.steps, args.warmup
 
