@@ -156,6 +156,21 @@ def test_american_requests_share_persistent_launches(engine):
     assert max(sizes) >= 4
 
 
+def test_served_american_basis_requests(engine):
+    """K-term regressions are served like any American request: same result as
+    the direct call, alongside quadratic ones."""
+    cases = [(K, N, s) for K, N, s in ((4, 1 << 16, 3), (0, 1 << 16, 3), (5, 1 << 15, 4), (2, 1 << 17, 5))]
+    alone = [engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, 50, "put", seed=s, lsm_basis=K) for K, N, s in cases]
+    with Server(engine, max_inflight=3) as srv:
+        say("served basis", [c[0] for c in cases])
+        tickets = [srv.submit("amer", 100, 100, 0.06, 0.2, 1.0, N, "put", M=50, seed=s, lsm_basis=K)
+                   for K, N, s in cases]
+        served = [srv.wait(t) for t in tickets]
+    for c, a, b in zip(cases, alone, served):
+        assert a.price == b.price and a.std_err == b.std_err, (c, a.price, b.price)
+    assert alone[0].price != alone[1].price  # the 4-term fit is a different estimator
+
+
 def test_served_fp32_within_se_of_fp64(engine):
     reqs = mixed_requests(5, 20, 16, 18)
     with Server(engine, max_inflight=4) as srv:
