@@ -89,8 +89,8 @@ typedef struct pcf_opts {
                                 set, give its block size and pairs per thread) */
     int32_t lsm_basis;       /* mc_amer: regression basis size K; 0 = 3 ({1, S/E, (S/E)^2}, or the
                                 reference's {1, S-E, (S-E)^2} with ref_exact); 2..6 = {1, x, ..,
-                                x^(K-1)}, x = S/E - 1 (production arithmetic, persistent form:
-                                M <= 1023 and a co-resident grid, else PCF_ERR_INVALID_ARG) */
+                                x^(K-1)}, x = S/E - 1 (production arithmetic; persistent and
+                                graph forms) */
     int32_t reserved[4];
 } pcf_opts;
 

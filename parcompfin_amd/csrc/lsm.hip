@@ -413,20 +413,40 @@ __device__ __forceinline__ void lsm_poly_decide(const LsmParams& P, const double
     }
 }
 
+// Normal equations from grid totals tot.v[off..] -> coef[date]: the quadratic
+// (lsm_solve) for K = 3, the K-term basis otherwise.  Runs in one thread.
+template <bool REF, int K, int NT>
+__device__ void lsm_solve_tot(const pcf::Acc<NT>& tot, int off, double* __restrict__ coef, int date) {
+    if constexpr (K == 3) {
+        pcf::Acc<8> g;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g.v[k] = tot.v[off + k];
+        lsm_solve<REF>(g, coef, date);
+    } else {
+        double g[LsmBasis<K>::kSums], c[K + 1];
+#pragma unroll
+        for (int k = 0; k < LsmBasis<K>::kSums; ++k) g[k] = tot.v[off + k].s + tot.v[off + k].c;
+        lsm_solve_poly<K>(g, c);
+#pragma unroll
+        for (int k = 0; k <= K; ++k) coef[LsmBasis<K>::kCoef * date + k] = c[k];
+    }
+}
+
 // --- k_lsm_paths ------------------------------------------------------------
 // Verification store: host normals w[n*M + (m-1)] (pathsfinder's draw order),
 // the reference's recurrence in FP64 without contraction.
-template <bool REF>
+template <bool REF, int K = 3>
 __global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
     using Real = double;
-    __shared__ pcf::CSum lds[4 * 10];
+    constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
+    __shared__ pcf::CSum lds[4 * (NS + 2)];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
     Real* __restrict__ stv = (Real*)b.st;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[NS] = {};
     double e0 = 0.0, e1 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < half; n += stride) {
@@ -457,38 +477,41 @@ __global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
         e0 += (double)cp + (double)cm_;
         e1 += (double)cp * (double)cp + (double)cm_ * (double)cm_;
         if (M >= 2) {
-            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
-            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+            if constexpr (K == 3) {
+                lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
+                lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+            } else {
+                lsm_poly_x<Real, K>(P, Sp_prev, acc);
+                lsm_poly_y<Real, K>(P, b.disc, M - 1, Sp_prev, M, cp, acc + LsmBasis<K>::kMom);
+                lsm_poly_x<Real, K>(P, Sm_prev, acc);
+                lsm_poly_y<Real, K>(P, b.disc, M - 1, Sm_prev, M, cm_, acc + LsmBasis<K>::kMom);
+            }
         }
     }
-    pcf::Acc<10> a, tot;
+    pcf::Acc<NS + 2> a, tot;
     a.v[0] = {e0, 0.0};
     a.v[1] = {e1, 0.0};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
+    for (int k = 0; k < NS; ++k) a.v[2 + k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<NS + 2>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         b.out[0] = tot.v[0].s + tot.v[0].c;
         b.out[1] = tot.v[1].s + tot.v[1].c;
-        if (M >= 2) {
-            pcf::Acc<8> g;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) g.v[k] = tot.v[2 + k];
-            lsm_solve<REF>(g, b.coef, M - 1);
-        }
+        if (M >= 2) lsm_solve_tot<REF, K>(tot, 2, b.coef, M - 1);
     }
 }
 
 // Production store: one Philox block (pair n, block (m-1)/4) feeds 4 dates.
-template <typename Real, bool REF>
+template <typename Real, bool REF, int K = 3>
 __global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
-    __shared__ pcf::CSum lds[4 * 10];
+    constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
+    __shared__ pcf::CSum lds[4 * (NS + 2)];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
     Real* __restrict__ stv = (Real*)b.st;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[NS] = {};
     double e0 = 0.0, e1 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < half; n += stride) {
@@ -527,61 +550,75 @@ __global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
         e0 += (double)cp + (double)cm_;
         e1 += (double)cp * (double)cp + (double)cm_ * (double)cm_;
         if (M >= 2) {
-            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
-            lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+            if constexpr (K == 3) {
+                lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sp_prev, M, cp, acc);
+                lsm_accumulate<Real, REF>(P, b.disc, M - 1, Sm_prev, M, cm_, acc);
+            } else {
+                lsm_poly_x<Real, K>(P, Sp_prev, acc);
+                lsm_poly_y<Real, K>(P, b.disc, M - 1, Sp_prev, M, cp, acc + LsmBasis<K>::kMom);
+                lsm_poly_x<Real, K>(P, Sm_prev, acc);
+                lsm_poly_y<Real, K>(P, b.disc, M - 1, Sm_prev, M, cm_, acc + LsmBasis<K>::kMom);
+            }
         }
     }
-    pcf::Acc<10> a, tot;
+    pcf::Acc<NS + 2> a, tot;
     a.v[0] = {e0, 0.0};
     a.v[1] = {e1, 0.0};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a.v[2 + k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<10>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
+    for (int k = 0; k < NS; ++k) a.v[2 + k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<NS + 2>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0) {
         b.out[0] = tot.v[0].s + tot.v[0].c;
         b.out[1] = tot.v[1].s + tot.v[1].c;
-        if (M >= 2) {
-            pcf::Acc<8> g;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) g.v[k] = tot.v[2 + k];
-            lsm_solve<REF>(g, b.coef, M - 1);
-        }
+        if (M >= 2) lsm_solve_tot<REF, K>(tot, 2, b.coef, M - 1);
     }
 }
 
 // --- k_lsm_step(m): decide at m, sums at m-1 --------------------------------
-template <typename Real, bool REF>
+template <typename Real, bool REF, int K = 3>
 __global__ __launch_bounds__(256) void k_lsm_step(LsmBufs b, int m) {
-    __shared__ pcf::CSum lds[4 * 8];
+    constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
+    __shared__ pcf::CSum lds[4 * NS];
     const LsmParams& P = *b.P;
     const Real* __restrict__ store = (const Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
     Real* __restrict__ stv = (Real*)b.st;
     const int64_t N = P.N;
-    const double* __restrict__ cm = b.coef + 4 * m;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double* __restrict__ cm = b.coef + LsmBasis<K>::kCoef * m;
+    double acc[NS] = {};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
         int32_t when = b.when[n];
         Real cf = cfv[n];
-        Real st = REF ? stv[n] : cf;
         const Real S = store[(int64_t)(m - 1) * N + n];
-        if (lsm_decide<Real, REF>(P, b.disc, cm, m, S, when, cf, st)) {
-            b.when[n] = when;
-            cfv[n] = cf;
-            if (REF) stv[n] = st;
-        }
         const Real Sprev = store[(int64_t)(m - 2) * N + n];
-        lsm_accumulate<Real, REF>(P, b.disc, m - 1, Sprev, when, cf, acc);
+        if constexpr (K == 3) {
+            Real st = REF ? stv[n] : cf;
+            if (lsm_decide<Real, REF>(P, b.disc, cm, m, S, when, cf, st)) {
+                b.when[n] = when;
+                cfv[n] = cf;
+                if (REF) stv[n] = st;
+            }
+            lsm_accumulate<Real, REF>(P, b.disc, m - 1, Sprev, when, cf, acc);
+        } else {
+            const int32_t when0 = when;
+            lsm_poly_decide<Real, K>(P, b.disc, cm, m, S, when, cf);
+            if (when != when0) {
+                b.when[n] = when;
+                cfv[n] = cf;
+            }
+            lsm_poly_x<Real, K>(P, Sprev, acc);
+            lsm_poly_y<Real, K>(P, b.disc, m - 1, Sprev, when, cf, acc + LsmBasis<K>::kMom);
+        }
     }
-    pcf::Acc<8> a, tot;
+    pcf::Acc<NS> a, tot;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a.v[k] = {acc[k], 0.0};
-    if (pcf::grid_reduce<8>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0)
-        lsm_solve<REF>(tot, b.coef, m - 1);
+    for (int k = 0; k < NS; ++k) a.v[k] = {acc[k], 0.0};
+    if (pcf::grid_reduce<NS>(a, b.partials, b.counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0)
+        lsm_solve_tot<REF, K>(tot, 0, b.coef, m - 1);
 }
 
 // --- k_lsm_final: decide at date 1 (if M >= 2), sum discounted cash flows ----
-template <typename Real, bool REF>
+template <typename Real, bool REF, int K = 3>
 __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
     __shared__ pcf::CSum lds[4 * 2];
     const LsmParams& P = *b.P;
@@ -590,7 +627,7 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
     Real* __restrict__ stv = (Real*)b.st;
     const int64_t N = P.N;
     const int M = P.M;
-    const double* __restrict__ cm = b.coef + 4;
+    const double* __restrict__ cm = b.coef + LsmBasis<K>::kCoef;
     double s = 0.0, s2 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
@@ -599,10 +636,20 @@ __global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
         Real st = REF ? stv[n] : cf;
         if (M >= 2) {
             const Real S = store[n];  // row of date 1
-            if (lsm_decide<Real, REF>(P, b.disc, cm, 1, S, when, cf, st)) {
-                b.when[n] = when;
-                cfv[n] = cf;
-                if (REF) stv[n] = st;
+            if constexpr (K == 3) {
+                if (lsm_decide<Real, REF>(P, b.disc, cm, 1, S, when, cf, st)) {
+                    b.when[n] = when;
+                    cfv[n] = cf;
+                    if (REF) stv[n] = st;
+                }
+            } else {
+                const int32_t when0 = when;
+                lsm_poly_decide<Real, K>(P, b.disc, cm, 1, S, when, cf);
+                st = cf;
+                if (when != when0) {
+                    b.when[n] = when;
+                    cfv[n] = cf;
+                }
             }
         }
         double v;
@@ -1427,10 +1474,10 @@ std::mutex& persistent_mu() {
 // class replays the same captured graph with the same launch shape.
 struct GraphKey {
     int64_t cap;
-    int M, real, src, ref, grid_paths, grid_step, B;
+    int M, real, src, ref, grid_paths, grid_step, B, K;
     bool operator<(const GraphKey& o) const {
-        return std::tie(cap, M, real, src, ref, grid_paths, grid_step, B) <
-               std::tie(o.cap, o.M, o.real, o.src, o.ref, o.grid_paths, o.grid_step, o.B);
+        return std::tie(cap, M, real, src, ref, grid_paths, grid_step, B, K) <
+               std::tie(o.cap, o.M, o.real, o.src, o.ref, o.grid_paths, o.grid_step, o.B, o.K);
     }
 };
 
@@ -1491,7 +1538,7 @@ std::mutex& graph_mu() {
 // Builds the date sweep as an explicit graph (kernel nodes in a chain).  No
 // stream capture is involved, so other threads may use the device freely
 // while a graph is being built.
-template <typename Real, int SRC, bool REF>
+template <typename Real, int SRC, bool REF, int K = 3>
 hipError_t build_lsm_graph(const LsmBufs& b, int M, int grid_paths, int grid_step, int B,
                            hipGraph_t* out) {
     hipGraph_t g = nullptr;
@@ -1517,16 +1564,16 @@ hipError_t build_lsm_graph(const LsmBufs& b, int M, int grid_paths, int grid_ste
     };
     void* a0[] = {&bb};
     if constexpr (SRC == kHost)
-        e = add((const void*)k_lsm_paths_host<REF>, grid_paths, a0);
+        e = add((const void*)k_lsm_paths_host<REF, K>, grid_paths, a0);
     else
-        e = add((const void*)k_lsm_paths_philox<Real, REF>, grid_paths, a0);
+        e = add((const void*)k_lsm_paths_philox<Real, REF, K>, grid_paths, a0);
     for (size_t i = 0; e == hipSuccess && i < dates.size(); ++i) {
         void* as[] = {&bb, &dates[i]};
-        e = add((const void*)k_lsm_step<Real, REF>, grid_step, as);
+        e = add((const void*)k_lsm_step<Real, REF, K>, grid_step, as);
     }
     if (e == hipSuccess) {
         void* af[] = {&bb};
-        e = add((const void*)k_lsm_final<Real, REF>, grid_step, af);
+        e = add((const void*)k_lsm_final<Real, REF, K>, grid_step, af);
     }
     if (e != hipSuccess) {
         (void)hipGraphDestroy(g);
@@ -1534,6 +1581,16 @@ hipError_t build_lsm_graph(const LsmBufs& b, int M, int grid_paths, int grid_ste
     }
     *out = g;
     return hipSuccess;
+}
+
+// The K-term graphs (production arithmetic: FP32 or FP64 on Philox, FP64 on
+// host normals).
+template <int K>
+hipError_t build_lsm_graph_k(bool f64, int src, const LsmBufs& b, int M, int grid_paths, int grid_step, int B,
+                             hipGraph_t* out) {
+    if (!f64) return build_lsm_graph<float, kPhilox, false, K>(b, M, grid_paths, grid_step, B, out);
+    return src == kHost ? build_lsm_graph<double, kHost, false, K>(b, M, grid_paths, grid_step, B, out)
+                        : build_lsm_graph<double, kPhilox, false, K>(b, M, grid_paths, grid_step, B, out);
 }
 
 }  // namespace
@@ -1589,8 +1646,6 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     if (K != 3 && ref)
         return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                         "the reference-exact mode regresses on the reference's 3-term basis (lsm_basis %d)", K);
-    if (K != 3 && o.kernel_variant == 1)
-        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "a %d-term basis runs in the persistent form only", K);
     const int CS = K == 3 ? 4 : 8;  // per-date coefficient stride: flag + K coefficients
 
     // Parameter block (host copy -> device), discount table, buffers.
@@ -1634,7 +1689,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const int grid_paths = sh.grid_paths;
     const int grid_step = sh.grid_step;
     const int grid_max = std::max(grid_paths, grid_step);
-    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid_max * 10 * sizeof(pcf::CSum)));
+    // graph form: (2 + sums) compensated partials per block (10 for the quadratic)
+    PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid_max * std::max(10, 3 * K + 1) * sizeof(pcf::CSum)));
 
     LsmBufs b{};
     b.store = ctx->lsm_store.ptr;
@@ -1679,7 +1735,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     // opts, else lsm_persistent_shape.
     bool persistent = false;
     int pgrid = 0, pmax = 0, PB = 0;
-    if ((o.kernel_variant == 2 || K != 3) && M + 1 > kDiscLds)
+    if (o.kernel_variant == 2 && M + 1 > kDiscLds)
         return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "persistent LSM form holds at most %d dates (M=%d)", kDiscLds - 1,
                         M);
     if (o.kernel_variant != 1 && M + 1 <= kDiscLds) {
@@ -1717,7 +1773,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             pmax = p2;
             PB = bs;
         }
-        if ((o.kernel_variant == 2 || K != 3) && !persistent)
+        if (o.kernel_variant == 2 && !persistent)
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                             "persistent LSM form does not fit N=%lld (%lld blocks of %d, at most %d co-resident)",
                             (long long)N, (long long)g, bs, gmax);
@@ -1772,7 +1828,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     }
 
     // Graph per shape; rebuilt if a buffer moved.
-    GraphKey key{sh.cap, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B};
+    GraphKey key{sh.cap, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B, K};
     hipGraphExec_t exec = nullptr;
     if (!persistent) {
         std::lock_guard<std::mutex> gl(graph_mu());
@@ -1785,7 +1841,11 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
             g = LsmGraph{};
             hipGraph_t graph = nullptr;
             hipError_t ce;
-            if (f64) {
+            if (K == 2) ce = build_lsm_graph_k<2>(f64, src, b, M, grid_paths, grid_step, B, &graph);
+            else if (K == 4) ce = build_lsm_graph_k<4>(f64, src, b, M, grid_paths, grid_step, B, &graph);
+            else if (K == 5) ce = build_lsm_graph_k<5>(f64, src, b, M, grid_paths, grid_step, B, &graph);
+            else if (K == 6) ce = build_lsm_graph_k<6>(f64, src, b, M, grid_paths, grid_step, B, &graph);
+            else if (f64) {
                 if (src == kHost)
                     ce = ref ? build_lsm_graph<double, kHost, true>(b, M, grid_paths, grid_step, B, &graph)
                              : build_lsm_graph<double, kHost, false>(b, M, grid_paths, grid_step, B, &graph);

@@ -318,15 +318,56 @@ def test_amer_basis_fp32_at_full_size(engine, K):
     assert abs(f32.price - q.price) < 0.02  # more terms move the estimate by little
 
 
+@pytest.mark.parametrize("K,N,prec", [(2, 1 << 16, "fp32k"), (4, 1 << 16, "fp64"), (6, 1 << 18, "fp32k"),
+                                       (5, 1 << 20, "fp32k")])
+def test_amer_basis_persistent_matches_graph_form(engine, K, N, prec):
+    """The K-term regression in the per-date graph and in the persistent sweep:
+    the same exercise dates up to a flip at the rounding boundary of a fit."""
+    M = 50
+    say("amer basis persistent vs graph", K, N, prec)
+    kw = dict(seed=31, precision=prec, lsm_basis=K, return_when=True)
+    g = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", kernel_variant=1, **kw)
+    p = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", kernel_variant=2, **kw)
+    flips = int((g.extra["when"] != p.extra["when"]).sum())
+    say("  flips", flips, "kernel_ms graph/persistent", g.kernel_ms, p.kernel_ms)
+    assert flips <= max(2, N // 100000)
+    if flips == 0:
+        assert abs(g.price - p.price) <= 1e-12 * g.price
+    else:
+        assert abs(g.price - p.price) <= 0.01 * g.std_err
+    assert g.lsm_fallback == p.lsm_fallback
+
+
+def test_amer_basis_graph_form_host_normals_vs_oracle(engine):
+    N, M, K = 8192, 50, 4
+    w = O.normals_mt(91, math.sqrt(1.0 / M), (N // 2) * M)
+    want, wwhen, wred = _amer_poly_ref(N, M, -1.0, w, K)
+    say("amer basis graph form, host normals")
+    got = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", normals=w, lsm_basis=K, kernel_variant=1,
+                         return_when=True)
+    flips = int((got.extra["when"] != wwhen).sum())
+    assert flips <= 2
+    assert rel(got.price, want) < (1e-11 if flips == 0 else 1e-4)
+    assert got.lsm_fallback == wred
+
+
+def test_amer_basis_many_dates(engine):
+    """M = 1100 exceeds the persistent form's date table: the graph form runs."""
+    say("amer basis M=1100")
+    r = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 16, 1100, "put", seed=6, lsm_basis=4)
+    eur = O.lib().orc_bs_price(100, 100, 0.06, 0.2, 1.0, -1.0)
+    assert eur < r.price and abs(r.price - 5.798877611783797) < 0.06
+    assert r.n_steps == 1100
+
+
 def test_amer_basis_errors(engine):
     from parcompfin_amd import PcfError
     say("amer basis errors")
-    for kw in (dict(lsm_basis=7), dict(lsm_basis=1), dict(lsm_basis=-2), dict(lsm_basis=4, ref_exact=True),
-               dict(lsm_basis=4, kernel_variant=1)):
+    for kw in (dict(lsm_basis=7), dict(lsm_basis=1), dict(lsm_basis=-2), dict(lsm_basis=4, ref_exact=True)):
         with pytest.raises(PcfError):
             engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", **kw)
-    with pytest.raises(PcfError):  # more dates than the persistent form holds
-        engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 1100, "put", lsm_basis=4)
+    with pytest.raises(PcfError):  # the persistent form asked for explicitly holds at most 1023 dates
+        engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 1100, "put", lsm_basis=4, kernel_variant=2)
     # 0 and 3 are the same quadratic; M = 1 has no regression date
     a = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", seed=4)
     b = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", seed=4, lsm_basis=3)
