@@ -12,13 +12,17 @@
 // no contraction, IEEE division, std::max's "a < b ? b : a", and u^i, d^k
 // from tables made on the host with glibc pow (the call the reference makes).
 //
-// Schedule: one launch advances kLatLevels levels.  Block b owns output nodes
-// [b W, b W + W) of the last of them; it loads the W + kLatLevels nodes of the
-// first level it needs into LDS and steps down level by level (ping-pong LDS
-// rows, one barrier per level).  The kLatLevels-wide right edge it forms is
-// also formed by its neighbour, with identical values.  Device data: two
-// level rows of N + 1 doubles and the tables u^i, d^k (i, k = 0..N), kept per
-// context for the last (u, d, N).
+// Schedule: one launch advances K = kLatLevels levels.  Each wavefront owns
+// Wout = 64 R - K output nodes of the last of them and holds the 64 R nodes
+// of the first level it needs in registers, R consecutive nodes per lane.
+// A level forms every node from itself and its right neighbour; the right
+// neighbour of a lane's last node is the next lane's first node, one DPP
+// wave shift away.  The span of valid nodes shrinks by one per level, so
+// after K levels the wave's first Wout nodes are exact and the waves need no
+// barrier or LDS at all.  The K-wide right edge a wave forms is also formed
+// by its neighbour, with identical values.  Device data: two level rows of
+// N + 1 doubles and the tables u^i, d^k (i, k = 0..N), kept per context for
+// the last (u, d, N).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -29,8 +33,9 @@
 namespace {
 
 constexpr int kLatThreads = 256;
-constexpr int kLatW = 512;        // output nodes per block and launch
-constexpr int kLatLevels = 128;   // levels per launch
+constexpr int kLatR = 4;                        // nodes per lane
+constexpr int kLatLevels = 128;                 // levels per launch
+constexpr int kLatWout = 64 * kLatR - kLatLevels;  // output nodes per wave and launch
 constexpr int64_t kLatMaxN = (int64_t)1 << 21;
 
 struct LatParams {
@@ -60,50 +65,76 @@ __global__ __launch_bounds__(kLatThreads) void k_lat_leaves(LatParams P, const d
     }
 }
 
+// Right neighbour across lanes: lane l receives lane l+1's value (DPP
+// wave_shl:1; lane 63 receives 0, which only ever reaches nodes past the
+// wave's valid span).  FP64 as two 32-bit halves.
+__device__ __forceinline__ double from_next_lane(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x130, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x130, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Levels n0-1 .. n0-K from level n0 (vin) into vout.  The grid covers the
-// n0-K+1 nodes of the last level; the host checks K <= kLatLevels.
+// n0-K+1 nodes of the last level with waves of kLatWout outputs; the host
+// checks K <= kLatLevels.  Every lane runs every level (no early exit), so the
+// cross-lane reads always see active lanes.
 template <bool AMER>
 __global__ __launch_bounds__(kLatThreads) void k_lat_steps(LatParams P, const double* __restrict__ up,
                                                            const double* __restrict__ dn,
                                                            const double* __restrict__ vin, double* __restrict__ vout,
                                                            int64_t n0, int K) {
-    __shared__ double ra[kLatW + kLatLevels], rb[kLatW + kLatLevels];
-    __shared__ double su[AMER ? kLatW + kLatLevels : 1];       // u^j, j = j0 + l
-    __shared__ double sd[AMER ? kLatW + 2 * kLatLevels : 1];   // d^k, k = kmin + t
-    const int64_t j0 = (int64_t)blockIdx.x * kLatW;
-    // Level n0 nodes this block needs: j0 .. j0 + W + K - 1 (clipped to n0).
-    const int nin = (int)(std::min<int64_t>(j0 + kLatW + K, n0 + 1) - j0);
-    for (int l = threadIdx.x; l < nin; l += blockDim.x) ra[l] = vin[j0 + l];
-    int64_t kmin = 0;
-    if constexpr (AMER) {
-        for (int l = threadIdx.x; l < nin; l += blockDim.x) su[l] = up[j0 + l];
-        // d^(n - j) for n in [n0-K, n0-1], j in [j0, j0 + W + K - 2], n - j >= 0.
-        kmin = std::max<int64_t>(0, n0 - j0 - kLatW - 2 * K + 2);
-        const int nd = (int)(n0 - 1 - j0 - kmin + 1);
-        for (int t = threadIdx.x; t < nd; t += blockDim.x) sd[t] = dn[kmin + t];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t j0 = wave * kLatWout + (int64_t)lane * kLatR;  // this lane's first node
+    // v: node values; AMER: u = u^j (fixed per node) and d = d^(n-j) for the
+    // level n being formed.  d^(n-j) of node j at level n-1 is d^(n-(j+1)) of
+    // node j+1 at level n, so d shifts one node left per level (the last node
+    // of a lane takes the next lane's first); only lane 63's last node takes a
+    // value it does not own, and that node is past the valid span already.
+    double v[kLatR], u[kLatR], d[kLatR];
+#pragma unroll
+    for (int r = 0; r < kLatR; ++r) {
+        const int64_t j = j0 + r;
+        v[r] = j <= n0 ? vin[j] : 0.0;
+        if constexpr (AMER) {
+            u[r] = j <= n0 ? up[j] : 0.0;
+            d[r] = j <= n0 - 1 ? dn[n0 - 1 - j] : 0.0;
+        }
     }
-    __syncthreads();
-    double* cur = ra;
-    double* nxt = rb;
     for (int s = 1; s <= K; ++s) {
         const int64_t n = n0 - s;
-        const int cnt = (int)(std::min<int64_t>(j0 + kLatW + K - s, n + 1) - j0);
-        for (int l = threadIdx.x; l < cnt; l += blockDim.x) {
+        const double nb = from_next_lane(v[0]);
+        double w[kLatR];
+#pragma unroll
+        for (int r = 0; r < kLatR; ++r) {
 #pragma clang fp contract(off)
-            double v = (P.p * cur[l + 1] + P.q * cur[l]) / P.R;
+            const double right = r + 1 < kLatR ? v[r + 1 < kLatR ? r + 1 : 0] : nb;
+            w[r] = (P.p * right + P.q * v[r]) / P.R;
             if constexpr (AMER) {
-                const double S = P.S0 * su[l] * sd[(n - (j0 + l)) - kmin];
-                v = ref_max(v, ref_payoff(S, P.E, P.pf));
+                const int64_t j = j0 + r;
+                if (j <= n) {
+                    const double S = P.S0 * u[r] * d[r];
+                    w[r] = ref_max(w[r], ref_payoff(S, P.E, P.pf));
+                }
             }
-            nxt[l] = v;
         }
-        __syncthreads();
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
+#pragma unroll
+        for (int r = 0; r < kLatR; ++r) v[r] = w[r];
+        if constexpr (AMER) {
+            const double dnext = from_next_lane(d[0]);
+#pragma unroll
+            for (int r = 0; r + 1 < kLatR; ++r) d[r] = d[r + 1];
+            d[kLatR - 1] = dnext;
+        }
     }
-    const int nout = (int)(std::min<int64_t>(j0 + kLatW, n0 - K + 1) - j0);
-    for (int l = threadIdx.x; l < nout; l += blockDim.x) vout[j0 + l] = cur[l];
+    // The wave's first kLatWout nodes of level n0-K (and nothing past it).
+    const int64_t wend = wave * kLatWout + kLatWout;
+#pragma unroll
+    for (int r = 0; r < kLatR; ++r) {
+        const int64_t j = j0 + r;
+        if (j < wend && j <= n0 - K) vout[j] = v[r];
+    }
 }
 
 }  // namespace
@@ -165,7 +196,8 @@ extern "C" int pcf_binom_vanilla(pcf_ctx* ctx, double S0, double E, double r, do
     for (int64_t n0 = N; n0 > 0;) {
         const int K = (int)std::min<int64_t>(kLatLevels, n0);
         const int64_t outs = n0 - K + 1;
-        const int grid = (int)((outs + kLatW - 1) / kLatW);
+        const int64_t waves = (outs + kLatWout - 1) / kLatWout;
+        const int grid = (int)((waves * 64 + kLatThreads - 1) / kLatThreads);
         if (american) k_lat_steps<true><<<grid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A, B, n0, K);
         else k_lat_steps<false><<<grid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A, B, n0, K);
         PCF_HIP_CHECK(ctx, hipGetLastError());
