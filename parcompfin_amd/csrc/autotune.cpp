@@ -111,8 +111,9 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
         targets.push_back({PCF_KIND_ASIA, 0, 1, grid({64, 128, 256, 512, 1024}, {1, 2}), {256, 1}});
     if (kind_mask & (1 << PCF_KIND_BASKET)) {
         // basket mode code = 2 * arithmetic mode + (MFMA ? 1 : 0): d = 8 on VALU, d = 32 on MFMA
-        targets.push_back({PCF_KIND_BASKET, 0, 8, grid({64, 128, 256}, {1, 2, 4}), {256, 1}});
-        targets.push_back({PCF_KIND_BASKET, 1, 32, grid({64, 128, 256}, {1, 2, 4, 8}), {256, 2}});
+        // ({0, 0}: the built-in shape, which grows the items per thread with N)
+        targets.push_back({PCF_KIND_BASKET, 0, 8, grid({64, 128, 256}, {1, 2, 4, 8, 16}), {0, 0}});
+        targets.push_back({PCF_KIND_BASKET, 1, 32, grid({64, 128, 256}, {1, 2, 4, 8, 16, 32}), {0, 0}});
     }
     if (kind_mask & (1 << PCF_KIND_AMER))
         // persistent LSM: (block size, pairs per thread); {0, 0} = built-in choice
@@ -160,7 +161,9 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
         }
         // Unmeasured classes take the nearest measured one (for the persistent
         // LSM shape only inside the measured range: beyond it the built-in
-        // choice adapts to the size and the graph form takes over).
+        // choice adapts to the size and the graph form takes over).  Above the
+        // measured range a tuned shape keeps its block count: the items per
+        // thread double with each class.
         if (measured.empty()) continue;
         std::lock_guard<std::mutex> tl(tmu());
         for (int c = 10; c < kClasses; ++c) {
@@ -168,8 +171,11 @@ extern "C" int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_pa
             for (int m : measured)
                 if (std::abs(m - c) < std::abs(nearest - c)) nearest = m;
             if (t.kind == PCF_KIND_AMER && (c < measured.front() || c > measured.back())) continue;
-            if (std::find(measured.begin(), measured.end(), c) == measured.end())
-                table()[t.kind][t.mode][c] = table()[t.kind][t.mode][nearest];
+            if (std::find(measured.begin(), measured.end(), c) == measured.end()) {
+                Shape s = table()[t.kind][t.mode][nearest];
+                if (s.B && c > nearest) s.ipt = std::min(1024, s.ipt << (c - nearest));
+                table()[t.kind][t.mode][c] = s;
+            }
         }
     }
     if (n_out) *n_out = std::min(nout, max_out);

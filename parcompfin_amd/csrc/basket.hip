@@ -75,11 +75,17 @@ struct BasketJob {
 };
 static_assert(sizeof(BasketJob) <= kJobBytes, "job record exceeds its staging slot");
 
+// The factor and the per-asset constants are read-only and the same for every
+// lane, so the FMA kernel reads them through the constant address space: the
+// compiler then issues scalar loads whose results feed the FMAs as SGPR
+// operands.  Through generic pointers every asset's row was a flat vector load
+// followed by a wait on the vector memory counter inside the path loop.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
 template <typename Real, int DMAX, int MODE, bool FULL>
-__device__ __forceinline__ void basket_fma_body(const BasketArgs& a, const Real* __restrict__ Lm,
-                                                const Real* __restrict__ c0,
-                                                const Real* __restrict__ c1,
-                                                const Real* __restrict__ ws,
+__device__ __forceinline__ void basket_fma_body(const BasketArgs& a, cptr<Real> Lm, cptr<Real> c0,
+                                                cptr<Real> c1, cptr<Real> ws,
                                                 const double* __restrict__ hz,
                                                 pcf::CSum* __restrict__ partials,
                                                 unsigned int* __restrict__ counter,
@@ -150,8 +156,8 @@ __global__ __launch_bounds__(256) void k_basket_fma(const BasketJob* __restrict_
     __shared__ pcf::CSum lds[4 * 2];
     const int j = pcf::find_job(jobs, njobs, blockIdx.x);
     const BasketJob& J = jobs[j];
-    basket_fma_body<Real, DMAX, MODE, FULL>(J.a, (const Real*)J.Lm, (const Real*)J.c0, (const Real*)J.c1,
-                                            (const Real*)J.ws, J.hz, J.partials, J.counter, J.out,
+    basket_fma_body<Real, DMAX, MODE, FULL>(J.a, (cptr<Real>)J.Lm, (cptr<Real>)J.c0, (cptr<Real>)J.c1,
+                                            (cptr<Real>)J.ws, J.hz, J.partials, J.counter, J.out,
                                             blockIdx.x - J.first_block, J.nblk, lds);
 }
 
@@ -442,18 +448,17 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
     a.N = q.N;
     a.d = d;
     a.seed = o.seed;
-    int B = 256, ipt = p.mfma ? 2 : 1;
+    // Built-in shape: 256 threads, at most 16 blocks per CU (the MFMA kernel's
+    // items are 32-path tiles, one wavefront per tile at a time).
+    const int64_t ntiles = (q.N + 31) / 32;
+    const int64_t max_blocks = 16LL * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+    int B = 256;
+    int ipt = p.mfma ? pcf_capped_ipt(ntiles, B / 64, 2, max_blocks) : pcf_capped_ipt(q.N, B, 1, max_blocks);
     pcf_tuned_shape(PCF_KIND_BASKET, p.mode * 2 + (p.mfma ? 1 : 0), q.N, &B, &ipt);
     if (o.block_size) B = o.block_size;
     if (o.items_per_thread) ipt = o.items_per_thread;
     p.B = B;
-    if (p.mfma) {
-        // ipt = 32-path tiles per wavefront
-        const int64_t ntiles = (q.N + 31) / 32;
-        p.nblk = pcf_blocks_for(ntiles, B / 64, ipt);
-    } else {
-        p.nblk = pcf_blocks_for(q.N, B, ipt);
-    }
+    p.nblk = p.mfma ? pcf_blocks_for(ntiles, B / 64, ipt) : pcf_blocks_for(q.N, B, ipt);
     p.disc = std::exp(-q.r * q.T);
     return PCF_OK;
 }
@@ -572,7 +577,7 @@ int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_resu
             noff += cnt;
         }
         J.partials = ctx->partials.as<pcf::CSum>() + poff;
-        J.counter = ctx->counters.as<unsigned int>() + k;
+        J.counter = ctx->counters.as<unsigned int>() + (size_t)k * pcf::kCounterWords;
         J.out = ctx->out.as<double>() + (size_t)kOutPerJob * k;
         J.first_block = boff;
         J.nblk = (unsigned int)p.nblk;

@@ -195,7 +195,7 @@ int pcf_eur_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result*
             noff += reqs[i]->N;
         }
         J.partials = ctx->partials.as<pcf::CSum>() + poff;
-        J.counter = ctx->counters.as<unsigned int>() + k;
+        J.counter = ctx->counters.as<unsigned int>() + (size_t)k * pcf::kCounterWords;
         J.out = ctx->out.as<double>() + (size_t)kOutPerJob * k;
         J.first_block = boff;
         J.nblk = (unsigned int)p.nblk;

@@ -214,8 +214,19 @@ __device__ __forceinline__ unsigned int fetch_u32(const unsigned int* p) {
 // Wait until every memory operation this wave issued is complete.
 __device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
 
+// Arrival counting of grid_reduce.  Device-scope atomics on one address run
+// one after another, so a single arrival counter costs nblk such atomics in
+// sequence at the end of a launch whose blocks finish together.  Block b
+// counts on group counter b mod kArriveGroups instead; the last block of each
+// group then counts on a top counter, whose last arrival is the last block
+// overall.  Each counter has its own 128-byte line; one reduction uses
+// kCounterWords uint32 of counter space.
+constexpr int kArriveGroups = 32;
+constexpr int kCounterLine = 32;  // uint32 per 128-byte line
+constexpr int kCounterWords = (kArriveGroups + 1) * kCounterLine;
+
 // Per-block partial write + "last block finishes" grid reduction.
-//   partials: nblk * K CSums;  counter: one uint32, zero at launch and left
+//   partials: nblk * K CSums;  counter: kCounterWords uint32, zero at launch and left
 //   zero on exit.  Returns true in the (single) block that performed the
 //   final pass, whose thread 0 then holds the totals in `tot`.
 //   blk / nblk: this block's index and the block count of the reduction (the
@@ -233,15 +244,40 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
             publish(&partials[(size_t)blk * K + k].c, a.v[k].c);
         }
         stores_complete();
-        const unsigned int prev = atomicAdd(counter, 1u);
-        is_last = (prev == nblk - 1);
+        // Group g holds blocks g, g + 32, ...: ceil((nblk - g) / 32) of them
+        // (blk < nblk, so g < nblk and the group is not empty).
+        const unsigned int g = blk % kArriveGroups;
+        const unsigned int ng = (nblk - g + kArriveGroups - 1) / kArriveGroups;
+        const unsigned int groups = nblk < (unsigned)kArriveGroups ? nblk : (unsigned)kArriveGroups;
+        bool last = false;
+        if (atomicAdd(&counter[g * kCounterLine], 1u) == ng - 1)
+            last = atomicAdd(&counter[kArriveGroups * kCounterLine], 1u) == groups - 1;
+        is_last = last;
     }
     __syncthreads();
     if (!is_last) return false;
     // Thread t folds partials t, t+B, t+2B, ... in order, then a block reduce.
+    // The loads of U consecutive partials are issued before their merges, so
+    // they are in flight together; the merge order is unchanged.
+    constexpr int U = K <= 2 ? 8 : K <= 4 ? 4 : 1;
     Acc<K> r;
     r.zero();
-    for (unsigned int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    unsigned int b = threadIdx.x;
+    for (; b + (U - 1) * blockDim.x < nblk; b += U * blockDim.x) {
+        CSum p[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                p[u][k].s = fetch(&partials[(size_t)(b + u * blockDim.x) * K + k].s);
+                p[u][k].c = fetch(&partials[(size_t)(b + u * blockDim.x) * K + k].c);
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < K; ++k) r.v[k] = csum_merge(r.v[k], p[u][k]);
+    }
+    for (; b < nblk; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             CSum p;
@@ -252,10 +288,11 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
     }
     __syncthreads();  // lds reuse
     r = block_reduce<K>(r, lds);
-    if (threadIdx.x == 0) {
-        tot = r;
-        publish_u32(counter, 0u);  // ready for the next launch
-    }
+    // Every block has counted (the top counter is complete): clear the
+    // counters for the next launch.
+    for (unsigned int c = threadIdx.x; c <= (unsigned)kArriveGroups; c += blockDim.x)
+        publish_u32(&counter[c * kCounterLine], 0u);
+    if (threadIdx.x == 0) tot = r;
     return true;
 }
 

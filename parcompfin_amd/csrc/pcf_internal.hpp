@@ -54,10 +54,10 @@ struct DevBuf {
     }
 };
 
-// Requests per batched launch; each owns one last-block counter and
-// kOutPerJob doubles of results.
+// Requests per batched launch; each owns the pcf::kCounterWords arrival
+// counters of one grid reduction and kOutPerJob doubles of results.
 constexpr int kMaxBatch = 64;
-constexpr int kCounterSlots = kMaxBatch;
+constexpr int kCounterSlots = kMaxBatch * pcf::kCounterWords;
 constexpr int kOutPerJob = 8;
 constexpr int kOutDoubles = kMaxBatch * kOutPerJob;
 // Bytes of one job record (launch descriptor) staged per batched launch.
@@ -82,7 +82,7 @@ struct pcf_ctx {
     char name[256] = {0};
 
     DevBuf partials;   // per-block compensated partial sums
-    DevBuf counters;   // kCounterSlots uint32 last-block counters, zero at rest
+    DevBuf counters;   // kCounterSlots uint32 arrival counters, zero at rest
     DevBuf out;        // kOutDoubles final values
     DevBuf normals;    // verification normals copied from the host
     DevBuf logtab;     // binom: glibc log(n), n = 0..N (resident per N)
@@ -144,6 +144,15 @@ static inline int pcf_blocks_for(int64_t items, int B, int ipt) {
     if (b < 1) b = 1;
     if (b > (1 << 20)) b = 1 << 20;
     return (int)b;
+}
+
+// Smallest power-of-two multiple of `ipt` (items per thread) that keeps a
+// grid-stride launch over `items` at B threads per block within max_blocks
+// blocks.  Beyond a few blocks per CU, extra blocks of a short kernel only
+// add dispatch work and arrivals at its final reduction.
+static inline int pcf_capped_ipt(int64_t items, int B, int ipt, int64_t max_blocks) {
+    while (ipt < 1024 && (items + (int64_t)B * ipt - 1) / ((int64_t)B * ipt) > max_blocks) ipt <<= 1;
+    return ipt;
 }
 
 // Size class of a request: ceil(log2 N) clamped to [10, 24].
