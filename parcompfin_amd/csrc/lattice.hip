@@ -12,7 +12,7 @@
 // no contraction, IEEE division, std::max's "a < b ? b : a", and u^i, d^k
 // from tables made on the host with glibc pow (the call the reference makes).
 //
-// Schedule: one launch advances K = kLatLevels levels.  Each wavefront owns
+// Schedule: one launch advances K levels (LatTile).  Each wavefront owns
 // Wout = 64 R - K output nodes of the last of them and holds the 64 R nodes
 // of the first level it needs in registers, R consecutive nodes per lane.
 // A level forms every node from itself and its right neighbour; the right
@@ -33,10 +33,22 @@
 namespace {
 
 constexpr int kLatThreads = 256;
-constexpr int kLatR = 4;                        // nodes per lane
-constexpr int kLatLevels = 128;                 // levels per launch
-constexpr int kLatWout = 64 * kLatR - kLatLevels;  // output nodes per wave and launch
 constexpr int64_t kLatMaxN = (int64_t)1 << 21;
+
+// Tile shape per exercise style: R nodes per lane, K levels per launch, Wout =
+// 64 R - K output nodes per wave and launch.  A level costs a wave about R
+// node updates in sequence, and a large lattice holds only ~n/Wout waves for
+// the 1024 SIMDs, so small tiles run faster per level than they lose to
+// redundant edges and extra launches.  Measured over 46 lattices (N = 513 ..
+// 10^5) against R = 4, K = 128: American R = 1, K = 32 171.5 -> 104.9 ms in
+// total; European R = 2, K = 64 69.3 -> 51.5 ms; bits unchanged.
+template <bool AMER>
+struct LatTile {
+    static constexpr int R = AMER ? 1 : 2;
+    static constexpr int K = AMER ? 32 : 64;
+    static constexpr int Wout = 64 * R - K;
+    static_assert(Wout > 0, "a wave must keep some exact output nodes");
+};
 
 struct LatParams {
     double S0, E, pf, p, q, R;
@@ -76,8 +88,8 @@ __device__ __forceinline__ double from_next_lane(double v) {
 }
 
 // Levels n0-1 .. n0-K from level n0 (vin) into vout.  The grid covers the
-// n0-K+1 nodes of the last level with waves of kLatWout outputs; the host
-// checks K <= kLatLevels.  Every lane runs every level (no early exit), so the
+// n0-K+1 nodes of the last level with waves of LatTile::Wout outputs; the
+// host keeps K <= LatTile::K.  Every lane runs every level (no early exit), so the
 // cross-lane reads always see active lanes.
 template <bool AMER>
 __global__ __launch_bounds__(kLatThreads) void k_lat_steps(LatParams P, const double* __restrict__ up,
@@ -86,15 +98,16 @@ __global__ __launch_bounds__(kLatThreads) void k_lat_steps(LatParams P, const do
                                                            int64_t n0, int K) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t j0 = wave * kLatWout + (int64_t)lane * kLatR;  // this lane's first node
+    constexpr int kR = LatTile<AMER>::R, kWout = LatTile<AMER>::Wout;
+    const int64_t j0 = wave * kWout + (int64_t)lane * kR;  // this lane's first node
     // v: node values; AMER: u = u^j (fixed per node) and d = d^(n-j) for the
     // level n being formed.  d^(n-j) of node j at level n-1 is d^(n-(j+1)) of
     // node j+1 at level n, so d shifts one node left per level (the last node
     // of a lane takes the next lane's first); only lane 63's last node takes a
     // value it does not own, and that node is past the valid span already.
-    double v[kLatR], u[kLatR], d[kLatR];
+    double v[kR], u[kR], d[kR];
 #pragma unroll
-    for (int r = 0; r < kLatR; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const int64_t j = j0 + r;
         v[r] = j <= n0 ? vin[j] : 0.0;
         if constexpr (AMER) {
@@ -105,11 +118,11 @@ __global__ __launch_bounds__(kLatThreads) void k_lat_steps(LatParams P, const do
     for (int s = 1; s <= K; ++s) {
         const int64_t n = n0 - s;
         const double nb = from_next_lane(v[0]);
-        double w[kLatR];
+        double w[kR];
 #pragma unroll
-        for (int r = 0; r < kLatR; ++r) {
+        for (int r = 0; r < kR; ++r) {
 #pragma clang fp contract(off)
-            const double right = r + 1 < kLatR ? v[r + 1 < kLatR ? r + 1 : 0] : nb;
+            const double right = r + 1 < kR ? v[r + 1 < kR ? r + 1 : 0] : nb;
             w[r] = (P.p * right + P.q * v[r]) / P.R;
             if constexpr (AMER) {
                 const int64_t j = j0 + r;
@@ -120,18 +133,18 @@ __global__ __launch_bounds__(kLatThreads) void k_lat_steps(LatParams P, const do
             }
         }
 #pragma unroll
-        for (int r = 0; r < kLatR; ++r) v[r] = w[r];
+        for (int r = 0; r < kR; ++r) v[r] = w[r];
         if constexpr (AMER) {
             const double dnext = from_next_lane(d[0]);
 #pragma unroll
-            for (int r = 0; r + 1 < kLatR; ++r) d[r] = d[r + 1];
-            d[kLatR - 1] = dnext;
+            for (int r = 0; r + 1 < kR; ++r) d[r] = d[r + 1];
+            d[kR - 1] = dnext;
         }
     }
-    // The wave's first kLatWout nodes of level n0-K (and nothing past it).
-    const int64_t wend = wave * kLatWout + kLatWout;
+    // The wave's first kWout nodes of level n0-K (and nothing past it).
+    const int64_t wend = wave * kWout + kWout;
 #pragma unroll
-    for (int r = 0; r < kLatR; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const int64_t j = j0 + r;
         if (j < wend && j <= n0 - K) vout[j] = v[r];
     }
@@ -193,10 +206,12 @@ extern "C" int pcf_binom_vanilla(pcf_ctx* ctx, double S0, double E, double r, do
     else k_lat_leaves<false><<<lgrid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A);
     PCF_HIP_CHECK(ctx, hipGetLastError());
     int launches = 1;
+    const int kmax = american ? LatTile<true>::K : LatTile<false>::K;
+    const int wout = american ? LatTile<true>::Wout : LatTile<false>::Wout;
     for (int64_t n0 = N; n0 > 0;) {
-        const int K = (int)std::min<int64_t>(kLatLevels, n0);
+        const int K = (int)std::min<int64_t>(kmax, n0);
         const int64_t outs = n0 - K + 1;
-        const int64_t waves = (outs + kLatWout - 1) / kLatWout;
+        const int64_t waves = (outs + wout - 1) / wout;
         const int grid = (int)((waves * 64 + kLatThreads - 1) / kLatThreads);
         if (american) k_lat_steps<true><<<grid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A, B, n0, K);
         else k_lat_steps<false><<<grid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A, B, n0, K);
