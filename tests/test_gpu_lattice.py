@@ -4,8 +4,9 @@ The bar is bit equality: every node is formed from its two children with the
 reference's operations (src/binom_vanilla_eur.cpp:32-36,
 src/binom_vanilla_amer.cpp:31-37), so the tiled schedule must give the
 oracle's bits, and the oracle's bits equal the reference drivers'
-(tests/golden/ref_kernels.csv).  N runs across the tile edges (128 levels and
-512 nodes per block and launch).
+(tests/golden/ref_kernels.csv).  N runs across the tile edges: American
+launches advance 32 levels with 32 exact nodes per wave (128 per block),
+European ones 64 levels with 64 per wave (256 per block).
 """
 import csv
 import os
@@ -23,7 +24,8 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-EDGES = [1, 2, 3, 64, 127, 128, 129, 255, 256, 511, 512, 513, 639, 640, 641, 1000, 2000, 5000]
+EDGES = [1, 2, 3, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 191, 192, 193, 255, 256, 257, 511, 512, 513,
+         639, 640, 641, 1000, 2000, 5000]
 
 
 @pytest.mark.parametrize("american", [0, 1])
@@ -100,3 +102,22 @@ def test_lattice_errors(engine):
         engine.binom_vanilla(100, 100, 0.06, 0.2, 1.0, (1 << 21) + 1, "put")
     with pytest.raises(PcfError):
         engine.binom_vanilla(100, 100, 0.06, 0.2, -1.0, 100, "put")
+
+
+def test_lattice_repeated_requests_bitwise(engine):
+    """Requests with the same N reuse the resident u^i, d^k tables and level
+    rows; calls alternate styles and parameters, with other work and
+    synchronisation between them, and each must still give the oracle's
+    bits."""
+    cases = [(100, 100, 0.06, 0.2, 1.0, "put", 1), (90, 100, 0.03, 0.35, 2.0, "call", 0),
+             (120, 100, 0.06, 0.2, 0.5, "put", 1), (100, 110, 0.02, 0.75, 1.0, "call", 0),
+             (100, 100, 0.06, 0.2, 1.0, "put", 1), (70, 100, 0.08, 0.5, 1.5, "put", 0)]
+    for N in (700, 4099):
+        for rep in range(2):
+            for S0, E, r, sg, T, pf, am in cases:
+                say("lattice repeat", N, rep, S0, E, pf, am)
+                got = engine.binom_vanilla(S0, E, r, sg, T, N, pf, american=bool(am)).price
+                want = O.lib().orc_binom_vanilla(S0, E, r, sg, T, N, 1.0 if pf == "call" else -1.0, am)
+                assert got == want, (N, S0, E, pf, am, got, want)
+                engine.mc_eur(100, 100, 0.05, 0.2, 1.0, 1 << 16, "call", seed=rep)  # other work between replays
+                engine.synchronize()
