@@ -16,7 +16,8 @@
 //    term is a prefix of one chain (log N + log(N-1) + ... in that order), so
 //    it comes from a host-made prefix table, rounding for rounding; only the
 //    second part runs per lane.  Its table index is uniform across a
-//    wavefront, so loads are scalar broadcasts (32 per wait); lanes that have
+//    wavefront, so loads are scalar broadcasts, 16 at a time, the next 16
+//    in flight while the current ones are subtracted; lanes that have
 //    reached their own end are masked, never reordered.
 //  * lgamma (production): lgamma(N+1) - lgamma(i+1) - lgamma(N-i+1) in FP64.
 #include <cmath>
@@ -64,12 +65,22 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     {
         int j = 2;
         const int hi_all = N - (w0 + pcf::kWave - 1);
-        for (; j + 31 <= hi_all; j += 32) {
-            double c[32];  // all 32 loads issued before the first subtraction waits
+        if (j + 15 <= hi_all) {
+            double c[16];
 #pragma unroll
-            for (int t = 0; t < 32; ++t) c[t] = tab[j + t];
+            for (int t = 0; t < 16; ++t) c[t] = tab[j + t];
+            for (; j + 31 <= hi_all; j += 16) {
+                double nx[16];  // the next 16 values load while these 16 are subtracted
 #pragma unroll
-            for (int t = 0; t < 32; ++t) acc -= c[t];
+                for (int t = 0; t < 16; ++t) nx[t] = tab[j + 16 + t];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) acc -= c[t];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) c[t] = nx[t];
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) acc -= c[t];
+            j += 16;
         }
         for (; j <= hi_all; ++j) acc -= tab[j];
         const int hi_any = N - w0;
