@@ -288,6 +288,17 @@ def profile_one(eng, which, K, W):
         "amer": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=s, precision=p),
     }
     out = {"profile": which}
+    if which == "lattice":
+        # binom_vanilla_eur call N=1e5 and binom_vanilla_amer put N=1e4 (FP64, reference bits)
+        legs = {"european_call_N1e5": lambda: eng.binom_vanilla(100, 100, 0.05, 0.2, 1.0, 10**5, "call"),
+                "american_put_N1e4": lambda: eng.binom_vanilla(100, 100, 0.06, 0.2, 1.0, 10**4, "put",
+                                                               american=True)}
+        for name, f in legs.items():
+            for _ in range(W):
+                f()
+            rs = [f() for _ in range(K)]
+            out[name] = {"kernel_ms": [r.kernel_ms for r in rs], "launches": rs[0].grid_blocks}
+        return out
     if which == "mix":
         from parcompfin_amd import Server
         mix = make_mix(2026, 64)
@@ -356,7 +367,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--profile", choices=["eur", "binom", "basket", "asia", "amer", "mix"],
+    ap.add_argument("--profile", choices=["eur", "binom", "basket", "asia", "amer", "mix", "lattice"],
                     help="run only one configuration --steps times in FP32/Kahan then FP64 "
                          "(for rocprofv3 kernel traces), print its timings and exit")
     args = ap.parse_args()
