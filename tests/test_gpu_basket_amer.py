@@ -89,6 +89,23 @@ def test_basket_fp64_philox_vs_oracle(engine, d):
     assert rel(got.price, want) < REL
 
 
+@pytest.mark.parametrize("nblk", [1, 33, 64, 65, 130])
+def test_basket_grid_reduction_across_arrival_groups(engine, nblk):
+    """Basket grids across the arrival groups of the grid reduction (64
+    threads per block, one path each): FP64 price equals the oracle's."""
+    d, B = 4, 64
+    N = nblk * B - 5
+    say("basket grid reduction, blocks", nblk)
+    Z = np.empty(N * d)
+    O.lib().orc_philox_basket_normals(23, N, d, O.ptr(Z))
+    L = chol(equicorr(d, 0.3))
+    want = O.lib().orc_basket_z(100, 100, 0.05, 0.2, 1.0, N, 1.0, d, O.ptr(L), O.ptr(Z), None, None, None)
+    got = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.3, seed=23, precision="fp64",
+                              block_size=B, items_per_thread=1)
+    assert got.grid_blocks == nblk
+    assert rel(got.price, want) < REL
+
+
 @pytest.mark.parametrize("d,variant", [(8, 0), (32, 1), (32, 2), (16, 2), (20, 2)])
 def test_basket_fp32_vs_fp64(engine, d, variant):
     say("basket fp32 vs fp64", d, variant)

@@ -108,6 +108,24 @@ def test_eur_deterministic_and_shape_invariant(engine):
     assert rel(a.price, c.price) < 1e-14  # same paths, different reduction tree
 
 
+@pytest.mark.parametrize("nblk", [1, 2, 31, 32, 33, 63, 64, 65, 97, 1000])
+def test_eur_grid_reduction_across_arrival_groups(engine, nblk):
+    """Grids whose block counts straddle the 32 arrival groups of the grid
+    reduction (block b counts on group b mod 32, the last of each group on a
+    top counter): every block's partial must reach the total, so the FP64
+    price equals the oracle's on the same Philox normals."""
+    B = 64  # threads per block, one item (4 paths) per thread
+    N = nblk * B * 4 - 3
+    say("mc_eur grid reduction, blocks", nblk, "paths", N)
+    se = C.c_double()
+    want = O.lib().orc_philox_mc_eur(100, 100, 0.05, 0.2, 1.0, N, 1.0, 13, C.byref(se))
+    got = engine.mc_eur(100, 100, 0.05, 0.2, 1.0, N, "call", seed=13, precision="fp64", block_size=B,
+                        items_per_thread=1)
+    assert got.grid_blocks == nblk
+    assert rel(got.price, want) < REL
+    assert rel(got.std_err, se.value) < 1e-9
+
+
 def test_eur_invalid_arguments(engine):
     from parcompfin_amd import PcfError
     say("mc_eur invalid arguments")
