@@ -135,6 +135,40 @@ __device__ __forceinline__ void lsm_accumulate(const LsmParams& P, const double*
     lsm_accumulate_y<Real, REF>(P, disc, m, S, when, cf, acc);
 }
 
+// The production quadratic's three passes over a date's path value -- the
+// x-moments (formed a date early), the y-terms and the decision -- share x =
+// S/E and the in-the-money test.  The persistent kernel keeps x from the
+// x-moments pass (-1 out of the money; x >= 0 in the money) and the other two
+// passes read it instead of repeating the payoff, the conversion and the
+// product.  Same operations on the same values, so the same bits.
+template <typename Real>
+__device__ __forceinline__ void lsm_accumulate_x_keep(const LsmParams& P, Real S, double acc[5], double& xo) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    xo = -1.0;
+    if (pv > 0.0) {
+        const double x = (double)S * P.invE;
+        const double x2 = x * x;
+        acc[0] += 1.0;
+        acc[1] += x;
+        acc[2] += x2;
+        acc[3] += x2 * x;
+        acc[4] += x2 * x2;
+        xo = x;
+    }
+}
+
+template <typename Real>
+__device__ __forceinline__ void lsm_accumulate_y_kept(const double* __restrict__ disc, int m, double xo,
+                                                      int32_t when, Real cf, double acc[8]) {
+    if (xo >= 0.0) {
+        const double y = disc[when - m] * (double)cf;
+        const double x2 = xo * xo;
+        acc[5] += y;
+        acc[6] += y * xo;
+        acc[7] += y * x2;
+    }
+}
+
 // Normal equations from the 8 totals g = {n, sx, sx2, sx3, sx4, sy, sxy,
 // sx2y} -> c = {flag, c0, c1, c2}.  Pure function of g: every thread that
 // calls it with the same totals gets the same coefficients.
@@ -278,6 +312,31 @@ __device__ __forceinline__ bool lsm_decide(const LsmParams& P, const double* __r
         return true;
     }
     return false;
+}
+
+// lsm_decide's production branch with x kept from the x-moments pass.
+template <typename Real>
+__device__ __forceinline__ void lsm_decide_kept(const LsmParams& P, const double* __restrict__ disc,
+                                                const double* __restrict__ cm, int m, Real S, double xo,
+                                                int32_t& when, Real& cf) {
+    const int flag = (int)cm[0];
+    if (flag != kFlagSmall && flag != kFlagRegress && flag != kFlagReduced) return;
+    if (!(xo >= 0.0)) return;
+    const Real pvr = pay<Real>(S, (Real)P.E, (Real)P.pf);
+    const double pv = (double)pvr;
+    if (flag == kFlagSmall) {
+        const double y = disc[when - m] * (double)cf;
+        if (pv > y) {
+            when = m;
+            cf = pvr;
+        }
+        return;
+    }
+    const double yhat = cm[1] + xo * (cm[2] + xo * cm[3]);
+    if (pv > yhat) {
+        when = m;
+        cf = pvr;
+    }
 }
 
 // ---- K-term basis {1, x, ..., x^(K-1)}, x = S/E - 1 (K = 2, 4, 5, 6) --------
@@ -1150,13 +1209,25 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             return rowbuf[((size_t)(m & 1) * 2 * PMAX + i) * blockDim.x + threadIdx.x];
         return row[i];
     };
+    // x kept between the passes (production quadratic, up to 4 pairs per
+    // thread: at 8 the extra registers would spill): xn for the date whose
+    // x-moments were formed last, xc for the date being processed.
+    constexpr bool kKeepX = K == 3 && !REF && PMAX <= 4;
+    double xc[kKeepX ? 2 * PMAX : 1], xn[kKeepX ? 2 * PMAX : 1];
     auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
         for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
+        if constexpr (kKeepX) {
+#pragma unroll
+            for (int i = 0; i < 2 * PMAX; ++i) xn[i] = -1.0;
+        }
 #pragma unroll
         for (int k = 0; k < PMAX; ++k)
             if (t0 + (int64_t)k * T < half) {
-                if constexpr (K == 3) {
+                if constexpr (kKeepX) {
+                    lsm_accumulate_x_keep<Real>(P, sval(row, m, 2 * k), xm, xn[2 * k]);
+                    lsm_accumulate_x_keep<Real>(P, sval(row, m, 2 * k + 1), xm, xn[2 * k + 1]);
+                } else if constexpr (K == 3) {
                     lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k), xm);
                     lsm_accumulate_x<Real, REF>(P, sval(row, m, 2 * k + 1), xm);
                 } else {
@@ -1178,6 +1249,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     unsigned int gen = 0;
     for (int m = M - 1; m >= 1; --m) {
         ++gen;
+        if constexpr (kKeepX) {
+#pragma unroll
+            for (int i = 0; i < 2 * PMAX; ++i) xc[i] = xn[i];
+        }
         if constexpr (kLdsRow) {
             if (m > 1) lsm_stage_row<Real, PMAX>(rowbuf, store, m - 1, N, half, T);
         } else if constexpr (!kPrefetch) {
@@ -1190,9 +1265,14 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             for (int k = 0; k < PMAX; ++k) {
                 const int64_t n = t0 + (int64_t)k * T;
                 if (n < half) {
-                    lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc);
-                    lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1],
-                                                cf[2 * k + 1], acc);
+                    if constexpr (kKeepX) {
+                        lsm_accumulate_y_kept<Real>(disc, m, xc[2 * k], when[2 * k], cf[2 * k], acc);
+                        lsm_accumulate_y_kept<Real>(disc, m, xc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
+                    } else {
+                        lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k), when[2 * k], cf[2 * k], acc);
+                        lsm_accumulate_y<Real, REF>(P, disc, m, sval(Sc, m, 2 * k + 1), when[2 * k + 1],
+                                                    cf[2 * k + 1], acc);
+                    }
                 }
             }
             lsm_publish(block_sum8(acc, red_a), slots_m, sy);
@@ -1242,9 +1322,13 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             const int i = 2 * k + h;
-                            Real stv = REF ? stb[REF ? i : 0] : cf[i];
-                            (void)lsm_decide<Real, REF>(P, disc, cm, m, sval(Sc, m, i), when[i], cf[i], stv);
-                            if constexpr (REF) stb[i] = stv;
+                            if constexpr (kKeepX) {
+                                lsm_decide_kept<Real>(P, disc, cm, m, sval(Sc, m, i), xc[i], when[i], cf[i]);
+                            } else {
+                                Real stv = REF ? stb[REF ? i : 0] : cf[i];
+                                (void)lsm_decide<Real, REF>(P, disc, cm, m, sval(Sc, m, i), when[i], cf[i], stv);
+                                if constexpr (REF) stb[i] = stv;
+                            }
                         }
                     }
                 }
