@@ -20,6 +20,11 @@
 //     --comparison X    value subtracted in the Abs_Error/Error columns (the
 //                       reference compiles it in via include/comparison.h; default 0;
 //                       the PCF_COMPARISON environment variable also works)
+//     --comparison-file H   read it from a header of that form instead (the line
+//                       "double comparison = X;" the runscripts write before they
+//                       rebuild, runscript_mc_eur.sh:10-11); also PCF_COMPARISON_FILE
+//     --header          print the runscripts' CSV header line before the row
+//                       (runscript_mc_eur.sh:13)
 //     --stats           one extra line on stderr: SE, kernel time, control values
 // A trailing plain integer (the _omp programs' thread count) is accepted and ignored.
 // Errors: message on stderr, exit status 1 (the reference aborts with an exception).
@@ -58,6 +63,7 @@ struct Args {
     bool ref_exact = false;
     bool production = false;
     double comparison = 0;
+    bool header = false;
     bool stats = false;
     std::vector<std::string> extra;  // program-specific --key value pairs
 };
@@ -85,6 +91,27 @@ inline long long inum(const char* s, const char* what) {
     }
 }
 
+// The value of `comparison` in a header such as the runscripts write into
+// include/comparison.h ("#pragma once" and "double comparison = X;",
+// runscript_mc_eur.sh:10-11).  Comments are not parsed; the first
+// "comparison" followed by '=' gives the value.
+inline double comparison_from_header(const char* path) {
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) die(std::string("cannot read comparison header '") + path + "'");
+    std::string text;
+    char buf[4096];
+    for (std::size_t n; (n = std::fread(buf, 1, sizeof buf, f)) > 0;) text.append(buf, n);
+    std::fclose(f);
+    for (std::size_t at = text.find("comparison"); at != std::string::npos; at = text.find("comparison", at + 1)) {
+        std::size_t eq = text.find_first_not_of(" \t\r\n", at + 10);
+        if (eq == std::string::npos || text[eq] != '=') continue;
+        const std::size_t end = text.find(';', eq);
+        if (end == std::string::npos) break;
+        return num(text.substr(eq + 1, end - eq - 1).c_str(), "comparison in the header");
+    }
+    die(std::string("no 'comparison = value;' in '") + path + "'");
+}
+
 // npos: number of positional arguments after the program name (7, 8 or 9).
 inline Args parse(int argc, char** argv, int npos, const char* usage) {
     Args a;
@@ -106,6 +133,7 @@ inline Args parse(int argc, char** argv, int npos, const char* usage) {
             a.rho = num(argv[9], "rho");
         }
     }
+    if (const char* c = std::getenv("PCF_COMPARISON_FILE")) a.comparison = comparison_from_header(c);
     if (const char* c = std::getenv("PCF_COMPARISON")) a.comparison = num(c, "PCF_COMPARISON");
     for (int i = 1 + npos; i < argc; ++i) {
         std::string f = argv[i];
@@ -131,6 +159,10 @@ inline Args parse(int argc, char** argv, int npos, const char* usage) {
             a.production = true;
         } else if (f == "--comparison") {
             a.comparison = num(val("--comparison"), "--comparison");
+        } else if (f == "--comparison-file") {
+            a.comparison = comparison_from_header(val("--comparison-file"));
+        } else if (f == "--header") {
+            a.header = true;
         } else if (f == "--stats") {
             a.stats = true;
         } else if (f == "--control") {
@@ -180,6 +212,9 @@ using clock_t_ = std::chrono::system_clock;
 // One row, formatted as include/common.h:231-248 does.
 inline void report(const Args& a, const char* method, double t_overall, double t_calc, double result,
                    int M, int assets) {
+    if (a.header)
+        std::cout << "Method,Payoff,S0,E,r,sigma,T,N,M,Parallel,Nr_of_assets,T_overall,T_calculation,Result,"
+                     "Abs_Error,Error\n";
     std::cout << std::setprecision(10) << method << "," << a.payoff << "," << a.S0 << "," << a.E << ","
               << a.r << "," << a.sigma << "," << a.T << "," << a.N << "," << M << "," << 1 << ","
               << assets << "," << t_overall << "," << t_calc << "," << result << ","

@@ -112,6 +112,40 @@ def test_comparison_column_from_the_environment():
     assert abs(float(got["Abs_Error"]) - abs(res - 10.45)) < 1e-8
 
 
+def test_comparison_header_errors(tmp_path):
+    # --comparison-file is read before any device is opened
+    p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 1000, "--comparison-file", str(tmp_path / "missing.h"))
+    assert p.returncode == 1 and "cannot read comparison header" in p.stderr
+    h = tmp_path / "comparison.h"
+    h.write_text("#pragma once\n")
+    p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 1000, "--comparison-file", str(h))
+    assert p.returncode == 1 and "no 'comparison = value;'" in p.stderr
+    h.write_text("#pragma once\ndouble comparison = ten;\n")
+    p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 1000, "--comparison-file", str(h))
+    assert p.returncode == 1 and "bad numeric argument" in p.stderr
+
+
+@pytest.mark.gpu
+def test_runscript_header_and_comparison_file(tmp_path):
+    # What the runscripts write before rebuilding (runscript_mc_eur.sh:10-13): the
+    # header line, and include/comparison.h with the comparison value.
+    h = tmp_path / "comparison.h"
+    h.write_text("#pragma once\ndouble comparison = 26.61316121;\n")
+    for how in (["--comparison-file", str(h)], []):
+        env = dict(os.environ, PCF_COMPARISON_FILE=str(h)) if not how else None
+        p = run("binom_embar", "call", 100, 110, 0.02, 0.75, 1, 10000, "--header", *how, env=env)
+        assert p.returncode == 0, p.stderr
+        lines = p.stdout.strip().splitlines()
+        assert len(lines) == 2 and lines[0] == ",".join(COLUMNS), lines
+        got = parse_row(lines[1])
+        res = float(got["Result"])
+        assert got["Result"] == "26.61316121"  # results/results_binom_embar.csv:16
+        # Error is the full-precision result minus the comparison (the row prints
+        # Result with 10 digits, so res - 26.61316121 is only good to ~1e-9)
+        assert abs(float(got["Error"]) - (res - 26.61316121)) < 1e-9
+        assert float(got["Abs_Error"]) == abs(float(got["Error"]))
+
+
 @pytest.mark.gpu
 def test_amer_basis_flag():
     # --basis K: the K-term regression; with --stats the fallback count is printed.
