@@ -363,7 +363,7 @@ def main():
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--latency-requests", type=int, default=400)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
