@@ -27,6 +27,9 @@
 namespace {
 
 constexpr int kMaxAssets = 64;
+// Threads per block of both basket kernels; their LDS reduction buffers are
+// sized for it and the host refuses larger blocks.
+constexpr int kBasketThreads = 256;
 enum { kFp32k = 0, kFp64 = 1, kHost = 2 };
 
 struct BasketArgs {
@@ -152,8 +155,8 @@ __device__ __forceinline__ void basket_fma_body(const BasketArgs& a, cptr<Real> 
 }
 
 template <typename Real, int DMAX, int MODE, bool FULL>
-__global__ __launch_bounds__(256) void k_basket_fma(const BasketJob* __restrict__ jobs, int njobs) {
-    __shared__ pcf::CSum lds[4 * 2];
+__global__ __launch_bounds__(kBasketThreads) void k_basket_fma(const BasketJob* __restrict__ jobs, int njobs) {
+    __shared__ pcf::CSum lds[(kBasketThreads / pcf::kWave) * 2];
     const int j = pcf::find_job(jobs, njobs, blockIdx.x);
     const BasketJob& J = jobs[j];
     basket_fma_body<Real, DMAX, MODE, FULL>(J.a, (cptr<Real>)J.Lm, (cptr<Real>)J.c0, (cptr<Real>)J.c1,
@@ -226,8 +229,8 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
         pcf::write_moments<1>(tot, a.N, out);
 }
 
-__global__ __launch_bounds__(256) void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
-    __shared__ pcf::CSum lds[4 * 2];
+__global__ __launch_bounds__(kBasketThreads) void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
+    __shared__ pcf::CSum lds[(kBasketThreads / pcf::kWave) * 2];
     const int j = pcf::find_job(jobs, njobs, blockIdx.x);
     const BasketJob& J = jobs[j];
     basket_mfma_body(J.a, (const float*)J.Lm, (const float*)J.c0, (const float*)J.c1, (const float*)J.ws,
@@ -457,6 +460,9 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
     pcf_tuned_shape(PCF_KIND_BASKET, p.mode * 2 + (p.mfma ? 1 : 0), q.N, &B, &ipt);
     if (o.block_size) B = o.block_size;
     if (o.items_per_thread) ipt = o.items_per_thread;
+    if (B > kBasketThreads)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "mc_eur_multi takes at most %d threads per block (block_size %d)",
+                        kBasketThreads, B);
     p.B = B;
     p.nblk = p.mfma ? pcf_blocks_for(ntiles, B / 64, ipt) : pcf_blocks_for(q.N, B, ipt);
     p.disc = std::exp(-q.r * q.T);

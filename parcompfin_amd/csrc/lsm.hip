@@ -494,11 +494,15 @@ __device__ void lsm_solve_tot(const pcf::Acc<NT>& tot, int off, double* __restri
 // --- k_lsm_paths ------------------------------------------------------------
 // Verification store: host normals w[n*M + (m-1)] (pathsfinder's draw order),
 // the reference's recurrence in FP64 without contraction.
+// Threads per block of the per-date (graph) form's kernels; their LDS
+// reduction buffers are sized for it and the host refuses larger blocks.
+constexpr int kLsmGraphThreads = 256;
+
 template <bool REF, int K = 3>
-__global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
+__global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_paths_host(LsmBufs b) {
     using Real = double;
     constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
-    __shared__ pcf::CSum lds[4 * (NS + 2)];
+    __shared__ pcf::CSum lds[(kLsmGraphThreads / pcf::kWave) * (NS + 2)];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
@@ -561,9 +565,9 @@ __global__ __launch_bounds__(256) void k_lsm_paths_host(LsmBufs b) {
 
 // Production store: one Philox block (pair n, block (m-1)/4) feeds 4 dates.
 template <typename Real, bool REF, int K = 3>
-__global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
+__global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_paths_philox(LsmBufs b) {
     constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
-    __shared__ pcf::CSum lds[4 * (NS + 2)];
+    __shared__ pcf::CSum lds[(kLsmGraphThreads / pcf::kWave) * (NS + 2)];
     const LsmParams& P = *b.P;
     Real* __restrict__ store = (Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
@@ -634,9 +638,9 @@ __global__ __launch_bounds__(256) void k_lsm_paths_philox(LsmBufs b) {
 
 // --- k_lsm_step(m): decide at m, sums at m-1 --------------------------------
 template <typename Real, bool REF, int K = 3>
-__global__ __launch_bounds__(256) void k_lsm_step(LsmBufs b, int m) {
+__global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_step(LsmBufs b, int m) {
     constexpr int NS = K == 3 ? 8 : LsmBasis<K>::kSums;
-    __shared__ pcf::CSum lds[4 * NS];
+    __shared__ pcf::CSum lds[(kLsmGraphThreads / pcf::kWave) * NS];
     const LsmParams& P = *b.P;
     const Real* __restrict__ store = (const Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
@@ -678,8 +682,8 @@ __global__ __launch_bounds__(256) void k_lsm_step(LsmBufs b, int m) {
 
 // --- k_lsm_final: decide at date 1 (if M >= 2), sum discounted cash flows ----
 template <typename Real, bool REF, int K = 3>
-__global__ __launch_bounds__(256) void k_lsm_final(LsmBufs b) {
-    __shared__ pcf::CSum lds[4 * 2];
+__global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_final(LsmBufs b) {
+    __shared__ pcf::CSum lds[(kLsmGraphThreads / pcf::kWave) * 2];
     const LsmParams& P = *b.P;
     const Real* __restrict__ store = (const Real*)b.store;
     Real* __restrict__ cfv = (Real*)b.cf;
@@ -1910,6 +1914,11 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         res->grid_blocks = pgrid;
         res->block_size = PB;
     }
+
+    if (!persistent && B > kLsmGraphThreads)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
+                        "the per-date LSM form takes at most %d threads per block (block_size %d)", kLsmGraphThreads,
+                        B);
 
     // Graph per shape; rebuilt if a buffer moved.
     GraphKey key{sh.cap, M, f64 ? 1 : 0, src, ref ? 1 : 0, grid_paths, grid_step, B, K};

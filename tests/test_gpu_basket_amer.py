@@ -150,6 +150,32 @@ def test_basket_invalid(engine):
         engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1000, "call", 8, 0.3, kernel_variant=2)
 
 
+def test_block_sizes_beyond_the_kernels_bound_are_refused(engine):
+    """The basket kernels and the LSM's per-date kernels are built for at most
+    256 threads per block: larger block sizes are refused before any launch,
+    and the context keeps working."""
+    from parcompfin_amd import PcfError
+    say("block sizes beyond the launch bound")
+    for B in (512, 1024):
+        with pytest.raises(PcfError) as ei:
+            engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 4096, "call", 8, 0.3, block_size=B)
+        assert ei.value.status == 1 and "threads per block" in str(ei.value)
+        with pytest.raises(PcfError) as ei:
+            engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", kernel_variant=1, block_size=B)
+        assert ei.value.status == 1 and "threads per block" in str(ei.value)
+    # the persistent form takes up to 1024 threads per block
+    r = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 4096, 20, "put", kernel_variant=2, block_size=1024,
+                       items_per_thread=1)
+    assert r.block_size == 1024
+    got = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 4096, "call", 4, 0.3, seed=23, precision="fp64",
+                              block_size=256, items_per_thread=1)
+    Z = np.empty(4096 * 4)
+    O.lib().orc_philox_basket_normals(23, 4096, 4, O.ptr(Z))
+    want = O.lib().orc_basket_z(100, 100, 0.05, 0.2, 1.0, 4096, 1.0, 4, O.ptr(chol(equicorr(4, 0.3))), O.ptr(Z),
+                                None, None, None)
+    assert rel(got.price, want) < REL
+
+
 # ---------------------------------------------------------------- mc_amer
 def _amer_ref(N, M, pf, w, quirk):
     when = np.zeros(N, dtype=np.int32)
