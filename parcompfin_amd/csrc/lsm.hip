@@ -1574,12 +1574,21 @@ struct LsmShape {
     int B, ipt, grid_paths, grid_step;
 };
 
-LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o) {
+// Built-in shape of the per-date (graph) form: 256 threads, at least 4 paths
+// per thread, and more until the grid is at most 2 blocks per CU (4 above 2^21
+// paths).  A date's kernel ends in a grid reduction of 8-10 sums whose last
+// block reads every block's partials, so its cost grows with the block count;
+// with too few blocks each thread's chain of dependent loads (state, then the
+// discount of its exercise date) is exposed instead.  Measured per request at
+// M = 50, FP32: 2^20 1.56 -> 1.25 ms (1024 -> 512 blocks), 2^22 4.66 -> 2.81
+// ms (4096 -> 1024), 2^23 8.28 -> 4.39 ms (8192 -> 1024).
+LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
     LsmShape sh;
     sh.cap = (int64_t)1 << pcf_size_class(N);
     if (sh.cap < N) sh.cap = N;  // beyond the largest class
     (void)mode;
-    int B = 256, ipt = 4;  // the tuned table holds the persistent form's shape
+    int B = 256;  // the tuned table holds the persistent form's shape
+    int ipt = pcf_capped_ipt(sh.cap, B, 4, (int64_t)(sh.cap <= ((int64_t)1 << 21) ? 2 : 4) * (cus > 0 ? cus : 256));
     if (o.block_size) B = o.block_size;
     if (o.items_per_thread) ipt = o.items_per_thread;
     sh.B = B;
@@ -1762,7 +1771,7 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     P.seed = o.seed;
 
     const int mode = f64 ? 1 : 0;
-    const LsmShape sh = lsm_shape(N, mode, o);
+    const LsmShape sh = lsm_shape(N, mode, o, ctx->cu_count);
     const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
     // coef (4 per date) | discount table | parameter block | sync words
