@@ -497,6 +497,8 @@ __device__ void lsm_solve_tot(const pcf::Acc<NT>& tot, int off, double* __restri
 // Threads per block of the per-date (graph) form's kernels; their LDS
 // reduction buffers are sized for it and the host refuses larger blocks.
 constexpr int kLsmGraphThreads = 256;
+// Paths whose loads a thread of the per-date kernels issues together.
+constexpr int kLsmBatch = 4;
 
 template <bool REF, int K = 3>
 __global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_paths_host(LsmBufs b) {
@@ -649,28 +651,49 @@ __global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_step(LsmBufs b, int m)
     const double* __restrict__ cm = b.coef + LsmBasis<K>::kCoef * m;
     double acc[NS] = {};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
-        int32_t when = b.when[n];
-        Real cf = cfv[n];
-        const Real S = store[(int64_t)(m - 1) * N + n];
-        const Real Sprev = store[(int64_t)(m - 2) * N + n];
-        if constexpr (K == 3) {
-            Real st = REF ? stv[n] : cf;
-            if (lsm_decide<Real, REF>(P, b.disc, cm, m, S, when, cf, st)) {
-                b.when[n] = when;
-                cfv[n] = cf;
-                if (REF) stv[n] = st;
+    // A thread's paths are taken kLsmBatch at a time: every load of the batch
+    // is issued before the first path is worked on, so the loads' latency is
+    // paid once per batch rather than once per path.  Paths are still summed
+    // in the order n, n + stride, n + 2 stride, ...
+    for (int64_t n0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n0 < N; n0 += kLsmBatch * stride) {
+        int32_t wv[kLsmBatch];
+        Real cv[kLsmBatch], Sv[kLsmBatch], Spv[kLsmBatch], sv[kLsmBatch];
+#pragma unroll
+        for (int u = 0; u < kLsmBatch; ++u) {
+            const int64_t n = n0 + u * stride;
+            const bool in = n < N;
+            wv[u] = in ? b.when[n] : 0;
+            cv[u] = in ? cfv[n] : (Real)0;
+            Sv[u] = in ? store[(int64_t)(m - 1) * N + n] : (Real)0;
+            Spv[u] = in ? store[(int64_t)(m - 2) * N + n] : (Real)0;
+            sv[u] = REF && in ? stv[n] : (Real)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLsmBatch; ++u) {
+            const int64_t n = n0 + u * stride;
+            if (n >= N) break;
+            int32_t when = wv[u];
+            Real cf = cv[u];
+            const Real S = Sv[u];
+            const Real Sprev = Spv[u];
+            if constexpr (K == 3) {
+                Real st = REF ? sv[u] : cf;
+                if (lsm_decide<Real, REF>(P, b.disc, cm, m, S, when, cf, st)) {
+                    b.when[n] = when;
+                    cfv[n] = cf;
+                    if (REF) stv[n] = st;
+                }
+                lsm_accumulate<Real, REF>(P, b.disc, m - 1, Sprev, when, cf, acc);
+            } else {
+                const int32_t when0 = when;
+                lsm_poly_decide<Real, K>(P, b.disc, cm, m, S, when, cf);
+                if (when != when0) {
+                    b.when[n] = when;
+                    cfv[n] = cf;
+                }
+                lsm_poly_x<Real, K>(P, Sprev, acc);
+                lsm_poly_y<Real, K>(P, b.disc, m - 1, Sprev, when, cf, acc + LsmBasis<K>::kMom);
             }
-            lsm_accumulate<Real, REF>(P, b.disc, m - 1, Sprev, when, cf, acc);
-        } else {
-            const int32_t when0 = when;
-            lsm_poly_decide<Real, K>(P, b.disc, cm, m, S, when, cf);
-            if (when != when0) {
-                b.when[n] = when;
-                cfv[n] = cf;
-            }
-            lsm_poly_x<Real, K>(P, Sprev, acc);
-            lsm_poly_y<Real, K>(P, b.disc, m - 1, Sprev, when, cf, acc + LsmBasis<K>::kMom);
         }
     }
     pcf::Acc<NS> a, tot;
@@ -693,38 +716,54 @@ __global__ __launch_bounds__(kLsmGraphThreads) void k_lsm_final(LsmBufs b) {
     const double* __restrict__ cm = b.coef + LsmBasis<K>::kCoef;
     double s = 0.0, s2 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
-        int32_t when = b.when[n];
-        Real cf = cfv[n];
-        Real st = REF ? stv[n] : cf;
-        if (M >= 2) {
-            const Real S = store[n];  // row of date 1
-            if constexpr (K == 3) {
-                if (lsm_decide<Real, REF>(P, b.disc, cm, 1, S, when, cf, st)) {
-                    b.when[n] = when;
-                    cfv[n] = cf;
-                    if (REF) stv[n] = st;
-                }
-            } else {
-                const int32_t when0 = when;
-                lsm_poly_decide<Real, K>(P, b.disc, cm, 1, S, when, cf);
-                st = cf;
-                if (when != when0) {
-                    b.when[n] = when;
-                    cfv[n] = cf;
+    for (int64_t n0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n0 < N; n0 += kLsmBatch * stride) {
+        int32_t wv[kLsmBatch];
+        Real cv[kLsmBatch], Sv[kLsmBatch], sv[kLsmBatch];
+#pragma unroll
+        for (int u = 0; u < kLsmBatch; ++u) {
+            const int64_t n = n0 + u * stride;
+            const bool in = n < N;
+            wv[u] = in ? b.when[n] : 0;
+            cv[u] = in ? cfv[n] : (Real)0;
+            Sv[u] = in && M >= 2 ? store[n] : (Real)0;  // row of date 1
+            sv[u] = REF && in ? stv[n] : (Real)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLsmBatch; ++u) {
+            const int64_t n = n0 + u * stride;
+            if (n >= N) break;
+            int32_t when = wv[u];
+            Real cf = cv[u];
+            Real st = REF ? sv[u] : cf;
+            if (M >= 2) {
+                const Real S = Sv[u];
+                if constexpr (K == 3) {
+                    if (lsm_decide<Real, REF>(P, b.disc, cm, 1, S, when, cf, st)) {
+                        b.when[n] = when;
+                        cfv[n] = cf;
+                        if (REF) stv[n] = st;
+                    }
+                } else {
+                    const int32_t when0 = when;
+                    lsm_poly_decide<Real, K>(P, b.disc, cm, 1, S, when, cf);
+                    st = cf;
+                    if (when != when0) {
+                        b.when[n] = when;
+                        cfv[n] = cf;
+                    }
                 }
             }
-        }
-        double v;
-        if (REF) {
+            double v;
+            if (REF) {
 #pragma clang fp contract(off)
-            // if(exercise_st[n]!=0) result+=exp(-r*exercise_when[n]*dt)*exercise_st[n];
-            v = ((double)st != 0.0) ? exp(P.nr * (double)when * P.dt) * (double)st : 0.0;
-        } else {
-            v = b.disc[when] * (double)st;
+                // if(exercise_st[n]!=0) result+=exp(-r*exercise_when[n]*dt)*exercise_st[n];
+                v = ((double)st != 0.0) ? exp(P.nr * (double)when * P.dt) * (double)st : 0.0;
+            } else {
+                v = b.disc[when] * (double)st;
+            }
+            s += v;
+            s2 += v * v;
         }
-        s += v;
-        s2 += v * v;
     }
     pcf::Acc<2> a, tot;
     a.v[0] = {s, 0.0};

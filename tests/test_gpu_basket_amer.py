@@ -282,7 +282,8 @@ def test_amer_errors_like_reference(engine):
 
 @pytest.mark.parametrize("N,M,prec,ref", [(1 << 16, 50, "fp32k", False), (1 << 16, 50, "fp64", False),
                                           (4096, 25, "fp64", True), (1 << 20, 50, "fp32k", False),
-                                          (1 << 22, 50, "fp32k", False), (2, 3, "fp64", False)])
+                                          (1 << 22, 50, "fp32k", False), (2, 3, "fp64", False),
+                                          (300002, 30, "fp32k", False), (300002, 30, "fp64", True)])
 def test_amer_persistent_matches_graph_form(engine, N, M, prec, ref):
     """The one-launch persistent sweep and the per-date graph give the same
     exercise dates (up to a flipped decision at the rounding boundary) and
