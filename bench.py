@@ -286,6 +286,8 @@ def profile_one(eng, which, K, W):
                                                seed=s, precision=p) for d, sg in ((8, sig8), (32, sig32))][-1],
         "asia": lambda p, s: eng.mc_asia(100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, "call", seed=s, precision=p),
         "amer": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, "put", seed=s, precision=p),
+        # 2^23 paths do not fit a co-resident grid: the per-date kernels
+        "amer_graph": lambda p, s: eng.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 23, 50, "put", seed=s, precision=p),
     }
     out = {"profile": which}
     if which == "lattice":
@@ -367,7 +369,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--profile", choices=["eur", "binom", "basket", "asia", "amer", "mix", "lattice"],
+    ap.add_argument("--profile", choices=["eur", "binom", "basket", "asia", "amer", "amer_graph", "mix", "lattice"],
                     help="run only one configuration --steps times in FP32/Kahan then FP64 "
                          "(for rocprofv3 kernel traces), print its timings and exit")
     args = ap.parse_args()
