@@ -1620,7 +1620,10 @@ struct LsmShape {
 // with too few blocks each thread's chain of dependent loads (state, then the
 // discount of its exercise date) is exposed instead.  Measured per request at
 // M = 50, FP32: 2^20 1.56 -> 1.25 ms (1024 -> 512 blocks), 2^22 4.66 -> 2.81
-// ms (4096 -> 1024), 2^23 8.28 -> 4.39 ms (8192 -> 1024).
+// ms (4096 -> 1024), 2^23 8.28 -> 4.39 ms (8192 -> 1024).  The paths kernel,
+// one thread per antithetic pair up to the cap and a grid reduction of 10 sums
+// at its end, is capped at 4 blocks per CU: 2^23 3.76 -> 3.45 ms (16384 ->
+// 1024 blocks).
 LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
     LsmShape sh;
     sh.cap = (int64_t)1 << pcf_size_class(N);
@@ -1632,7 +1635,7 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
     if (o.items_per_thread) ipt = o.items_per_thread;
     sh.B = B;
     sh.ipt = ipt;
-    sh.grid_paths = pcf_blocks_for(sh.cap / 2, B, 1);
+    sh.grid_paths = pcf_blocks_for(sh.cap / 2, B, pcf_capped_ipt(sh.cap / 2, B, 1, (int64_t)4 * (cus > 0 ? cus : 256)));
     sh.grid_step = pcf_blocks_for(sh.cap, B, ipt);
     return sh;
 }
