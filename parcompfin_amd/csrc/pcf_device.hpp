@@ -259,7 +259,7 @@ __device__ __forceinline__ bool grid_reduce(Acc<K> a, CSum* __restrict__ partial
     // Thread t folds partials t, t+B, t+2B, ... in order, then a block reduce.
     // The loads of U consecutive partials are issued before their merges, so
     // they are in flight together; the merge order is unchanged.
-    constexpr int U = K <= 2 ? 8 : K <= 4 ? 4 : 1;
+    constexpr int U = K <= 2 ? 8 : K <= 4 ? 4 : K <= 8 ? 2 : 1;
     Acc<K> r;
     r.zero();
     unsigned int b = threadIdx.x;
