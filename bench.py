@@ -33,7 +33,9 @@ cpu_baseline: the reference's own OpenMP programs (src/mc_eur_omp.cpp,
 src/mc_asia_omp.cpp, src/mc_amer_omp.cpp built from source at -O2 -fopenmp
 into oracle/_ref) on a bounded sample of the same mix (the reference basket
 program needs Eigen/Boost, absent here, so basket requests are left out of
-the sample); paths x steps / total T_calculation.
+the sample); paths x steps / total T_calculation, also per kind ("by_kind",
+repeated in the per-configuration objects as cpu_reference_*), and one run
+each of the reference's binomial programs ("binomial").
 """
 import argparse
 import json
@@ -320,6 +322,34 @@ def profile_one(eng, which, K, W):
 
 
 # ---------------------------------------------------------------- CPU baseline
+def cpu_binomial(ref, env, threads):
+    """One run each of the reference's binomial programs (built from src/ into
+    oracle/_ref), in the same units as the GPU's per-configuration figures:
+    binom_embar_omp at N = 10^5 (reference adds 0.75 N^2 per second),
+    binom_vanilla_amer at N = 10^4 and binom_vanilla_eur at N = 10^4 (node
+    updates (N+1)(N+2)/2 per second; the European lattice is timed at 10^4, not
+    the GPU's 10^5, because the serial program takes minutes there).  About
+    5-10 s of CPU time."""
+    legs = {"binom_embar_N100000": (["binom_embar_omp", "call", 100, 100, 0.05, 0.2, 1.0, 100000, threads],
+                                    lambda N, t: {"log_adds_per_s_chain": 0.75 * N * N / t}),
+            "lattice_amer_put_N1e4": (["binom_vanilla_amer", "put", 100, 100, 0.06, 0.2, 1.0, 10000],
+                                      lambda N, t: {"node_updates_per_s": 0.5 * (N + 1) * (N + 2) / t}),
+            "lattice_eur_call_N1e4": (["binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1.0, 10000],
+                                      lambda N, t: {"node_updates_per_s": 0.5 * (N + 1) * (N + 2) / t})}
+    out = {}
+    for name, (argv, rate) in legs.items():
+        exe = os.path.join(ref, argv[0])
+        if not os.path.exists(exe):
+            out[name] = None
+            continue
+        row = subprocess.run([exe] + [str(x) for x in argv[1:]], env=env, check=True, capture_output=True,
+                             text=True, timeout=600).stdout.strip().split(",")
+        t, N = float(row[12]), int(argv[7])
+        out[name] = dict({"seconds": t, "N": N, "price": float(row[13]),
+                          "threads": threads if argv[0].endswith("_omp") else 1}, **rate(N, t))
+    return out
+
+
 def cpu_baseline(mix, budget_s):
     threads = max(1, min(16, os.cpu_count() or 1))
     ref = os.path.join(ROOT, "oracle", "_ref")
@@ -345,11 +375,19 @@ def cpu_baseline(mix, budget_s):
     total_ps = path_steps(sample)
     scale = min(1.0, budget_s * rate0 / total_ps)
     tot_t, tot_ps = 0.0, 0
+    kinds = {}
     for q in sample:
         N = max(64, int(q["args"][5] * scale) // 2 * 2)
-        tot_t += run(q, N)
+        t = run(q, N)
+        tot_t += t
         tot_ps += N * q["steps"]
+        k = kinds.setdefault(q["kind"], [0.0, 0])
+        k[0] += t
+        k[1] += N * q["steps"]
+    by_kind = {k: {"paths_steps_per_s": v[1] / v[0], "paths_steps": v[1], "seconds": v[0]}
+               for k, v in kinds.items()}
     return {"value": tot_ps / tot_t, "unit": "paths·steps/s", "cores": threads, "kind": "reference",
+            "by_kind": by_kind, "binomial": cpu_binomial(ref, env, threads),
             "sample": f"{len(sample)} European/Asian/American requests of the served mix with path counts "
                       f"scaled by {scale:.4f} ({tot_ps:.3g} paths·steps), run by the reference's "
                       f"mc_eur_omp/mc_asia_omp/mc_amer_omp (built -O2 -fopenmp from src/), FP64, "
@@ -452,6 +490,15 @@ def main():
         try:
             cpu = cpu_baseline(mix, args.cpu_seconds)
             log("cpu baseline:", cpu)
+            # the CPU's rates beside the GPU's, per configuration
+            for name, kind in (("mc_eur_1e6", "eur"), ("mc_asia_2^22x252", "asia"), ("mc_amer_put_2^20x50", "amer")):
+                if name in configs and kind in cpu.get("by_kind", {}):
+                    configs[name]["cpu_reference_paths_steps_per_s"] = cpu["by_kind"][kind]["paths_steps_per_s"]
+                    configs[name]["cpu_reference_sample"] = "served-mix requests of this kind, scaled; see cpu_baseline"
+            for name, leg in (cpu.get("binomial") or {}).items():
+                key = "lattice_eur_call_N1e5" if name == "lattice_eur_call_N1e4" else name
+                if leg and key in configs:
+                    configs[key]["cpu_reference"] = leg
         except Exception as e:  # reported, never replaced by something else
             log("cpu baseline failed:", repr(e))
             cpu = {"value": None, "unit": "paths·steps/s", "cores": None, "kind": None, "sample": f"failed: {e!r}"}
