@@ -60,7 +60,10 @@ struct pcf_server {
     std::unordered_map<uint64_t, Finished> done;
     std::unordered_map<uint64_t, int> live;  // submitted, not yet collected
     int64_t budget = 0, in_use = 0;
-    int64_t resident_cap = 0;
+    // Path-store bytes that slots keep between American requests (per slot,
+    // and their total), bounded by the budget as a whole.
+    std::vector<int64_t> resident;
+    int64_t resident_total = 0;
     bool stop = false;
     uint64_t next_ticket = 1;
     clk::time_point t0;
@@ -86,7 +89,7 @@ int64_t footprint_of(const pcf_request& q, int64_t nblk_hint) {
 }
 
 // Runs one unit (items sharing a batch key, or one item) on a slot context.
-void run_unit(pcf_server* srv, pcf_ctx* ctx, std::vector<Item>& unit, std::vector<Finished>& fin) {
+void run_unit(pcf_server* srv, int slot, pcf_ctx* ctx, std::vector<Item>& unit, std::vector<Finished>& fin) {
     const int n = (int)unit.size();
     fin.assign(n, Finished{});
     std::vector<const pcf_request*> reqs(n);
@@ -108,12 +111,27 @@ void run_unit(pcf_server* srv, pcf_ctx* ctx, std::vector<Item>& unit, std::vecto
                             ? pcf_mc_amer(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.M, q.payoff_fun, &q.opts, &res[i])
                             : pcf_binom_embar(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun, &q.opts, &res[i]);
         }
-        // Keep large path stores from accumulating across idle slots.
-        if (kind == PCF_KIND_AMER && (int64_t)ctx->lsm_store.bytes > srv->resident_cap) {
-            std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-            pcf_lsm_release(ctx);
-            ctx->lsm_store.release();
-            ctx->lsm_state.release();
+        // A slot keeps its path store for the next American request unless
+        // the stores kept by all slots would exceed the budget; then this one
+        // is freed.  A fixed share per slot (budget / slots) would instead
+        // free and reallocate a store on every request once that share is
+        // smaller than one store.
+        if (kind == PCF_KIND_AMER) {
+            const int64_t held = (int64_t)(ctx->lsm_store.bytes + ctx->lsm_state.bytes);
+            bool release;
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                srv->resident_total += held - srv->resident[slot];
+                release = srv->resident_total > srv->budget;
+                if (release) srv->resident_total -= held;
+                srv->resident[slot] = release ? 0 : held;
+            }
+            if (release) {
+                std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+                pcf_lsm_release(ctx);
+                ctx->lsm_store.release();
+                ctx->lsm_state.release();
+            }
         }
     }
     const double t = srv->now_ms();
@@ -176,7 +194,7 @@ void worker_main(pcf_server* srv, int slot) {
             });
             if (unit.empty()) return;
         }
-        run_unit(srv, ctx, unit, fin);
+        run_unit(srv, slot, ctx, unit, fin);
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             for (size_t i = 0; i < unit.size(); ++i) {
@@ -201,7 +219,7 @@ extern "C" int pcf_server_create(pcf_ctx* ctx, const pcf_server_config* cfg, pcf
     auto* srv = new pcf_server();
     srv->device = ctx->device;
     srv->budget = c.mem_budget_bytes > 0 ? c.mem_budget_bytes : ((int64_t)8 << 30);
-    srv->resident_cap = srv->budget / nslots;
+    srv->resident.assign(nslots, 0);
     srv->t0 = clk::now();
     int st = pcf_ctx_create(ctx->device, &srv->front);
     if (st != PCF_OK) {
