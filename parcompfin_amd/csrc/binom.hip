@@ -46,6 +46,18 @@ __device__ __forceinline__ double term_value(const BinomArgs& a, double comb_val
 
 // Chain form.  One lane per term index; all 64 lanes of a wave run the same
 // trip counts (wave-uniform loop bounds from the wave's first index).
+//
+// STAGED, for grids of a few waves per SIMD, where the longest chain's
+// latency bounds the time: the block copies the table through LDS,
+// kStageChunk entries at a time (vector loads of the next chunk in flight
+// while the current one is subtracted, one barrier per chunk, the same chunk
+// count for every wave of the block), and each wave reads its values as LDS
+// broadcasts.  Scalar loads can return out of order, so a use waits for every
+// scalar load outstanding and their prefetch cannot run deeper than one
+// batch; LDS reads complete in order.  Same subtractions in the same order,
+// so the same bits.
+constexpr int kStageChunk = 1024;
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* __restrict__ tab,
                                                      const double* __restrict__ pre,
                                                      pcf::CSum* __restrict__ partials,
@@ -62,7 +74,55 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     const int64_t k1 = a.N - i;
     double acc = pre[k1 < 0 ? 0 : (k1 > a.N ? a.N : k1)];
     // Part 2: j = 2 .. N-i.  Unmasked while j <= N-(i0+63).
-    {
+    if constexpr (STAGED) {
+        __shared__ double stage[2][kStageChunk];
+        const int b0 = __builtin_amdgcn_readfirstlane((int)((int64_t)blockIdx.x * blockDim.x));
+        const int blk_hi = N - b0;                   // largest j of any lane of the block
+        const int hi_all = N - (w0 + pcf::kWave - 1);  // this wave: unmasked up to here
+        const int hi_any = N - w0;                   // masked up to here (lane i stops at N - i)
+        const int nchunks = blk_hi >= 2 ? (blk_hi - 2) / kStageChunk + 1 : 0;
+        constexpr int kPer = kStageChunk / 256;      // entries per thread per chunk
+        double v[kPer];
+        auto fetch = [&](int c) {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int e = 2 + c * kStageChunk + u * 256 + (int)threadIdx.x;
+                v[u] = tab[e < N ? e : N];
+            }
+        };
+        if (nchunks > 0) {
+            fetch(0);
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) stage[0][u * 256 + threadIdx.x] = v[u];
+        }
+        __syncthreads();
+        for (int c = 0; c < nchunks; ++c) {
+            const bool more = c + 1 < nchunks;
+            if (more) fetch(c + 1);
+            const double* S = stage[c & 1];
+            const int base = 2 + c * kStageChunk;
+            const int last = base + kStageChunk - 1;
+            const int e_all = last < hi_all ? last : hi_all;
+            int j = base;
+            for (; j + 15 <= e_all; j += 16) {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) acc -= S[j - base + t];
+            }
+            for (; j <= e_all; ++j) acc -= S[j - base];
+            const int e_any = last < hi_any ? last : hi_any;
+            for (j = j > hi_all + 1 ? j : hi_all + 1; j <= e_any; ++j) {
+                const double t = S[j - base];
+                acc = (j <= N - i) ? acc - t : acc;
+            }
+            if (more) {
+                // buffer (c+1)&1 was last read in chunk c-1, which every wave
+                // finished before the previous barrier
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) stage[(c + 1) & 1][u * 256 + threadIdx.x] = v[u];
+            }
+            __syncthreads();
+        }
+    } else {
         int j = 2;
         const int hi_all = N - (w0 + pcf::kWave - 1);
         if (j + 15 <= hi_all) {
@@ -190,10 +250,15 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
-        k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
-                                                  ctx->partials.as<pcf::CSum>(),
-                                                  ctx->counters.as<unsigned int>(),
-                                                  ctx->out.as<double>());
+    {
+        // few waves per SIMD (at most 2): the chain's latency bounds the time
+        const int64_t waves = (a.lanes + pcf::kWave - 1) / pcf::kWave;
+        const bool staged = waves <= 2LL * 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+        auto kern = staged ? k_binom_chain<true> : k_binom_chain<false>;
+        kern<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
+                                          ctx->partials.as<pcf::CSum>(), ctx->counters.as<unsigned int>(),
+                                          ctx->out.as<double>());
+    }
     else
         k_binom_lgamma<<<grid, B, 0, ctx->stream>>>(a, ctx->partials.as<pcf::CSum>(),
                                                    ctx->counters.as<unsigned int>(),
