@@ -16,9 +16,9 @@
 //    term is a prefix of one chain (log N + log(N-1) + ... in that order), so
 //    it comes from a host-made prefix table, rounding for rounding; only the
 //    second part runs per lane.  Its table index is uniform across a
-//    wavefront, so loads are scalar broadcasts, 16 at a time, the next 16
-//    in flight while the current ones are subtracted; lanes that have
-//    reached their own end are masked, never reordered.
+//    wavefront: the block stages the table through LDS and every wave reads
+//    its values as LDS broadcasts; lanes that have reached their own end are
+//    masked, never reordered.
 //  * lgamma (production): lgamma(N+1) - lgamma(i+1) - lgamma(N-i+1) in FP64.
 #include <cmath>
 #include <vector>
@@ -47,17 +47,14 @@ __device__ __forceinline__ double term_value(const BinomArgs& a, double comb_val
 // Chain form.  One lane per term index; all 64 lanes of a wave run the same
 // trip counts (wave-uniform loop bounds from the wave's first index).
 //
-// STAGED, for grids of a few waves per SIMD, where the longest chain's
-// latency bounds the time: the block copies the table through LDS,
-// kStageChunk entries at a time (vector loads of the next chunk in flight
-// while the current one is subtracted, one barrier per chunk, the same chunk
-// count for every wave of the block), and each wave reads its values as LDS
-// broadcasts.  Scalar loads can return out of order, so a use waits for every
-// scalar load outstanding and their prefetch cannot run deeper than one
-// batch; LDS reads complete in order.  Same subtractions in the same order,
-// so the same bits.
+// The block copies the table through LDS, kStageChunk entries at a time
+// (vector loads of the next chunk in flight while the current one is
+// subtracted, one barrier per chunk, the same chunk count for every wave of
+// the block), and each wave reads its values as LDS broadcasts.  Scalar loads
+// of the table can return out of order, so a use waits for every scalar load
+// outstanding and their prefetch cannot run deeper than one batch; LDS reads
+// complete in order.
 constexpr int kStageChunk = 1024;
-template <bool STAGED>
 __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* __restrict__ tab,
                                                      const double* __restrict__ pre,
                                                      pcf::CSum* __restrict__ partials,
@@ -74,7 +71,7 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     const int64_t k1 = a.N - i;
     double acc = pre[k1 < 0 ? 0 : (k1 > a.N ? a.N : k1)];
     // Part 2: j = 2 .. N-i.  Unmasked while j <= N-(i0+63).
-    if constexpr (STAGED) {
+    {
         __shared__ double stage[2][kStageChunk];
         const int b0 = __builtin_amdgcn_readfirstlane((int)((int64_t)blockIdx.x * blockDim.x));
         const int blk_hi = N - b0;                   // largest j of any lane of the block
@@ -121,32 +118,6 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
                 for (int u = 0; u < kPer; ++u) stage[(c + 1) & 1][u * 256 + threadIdx.x] = v[u];
             }
             __syncthreads();
-        }
-    } else {
-        int j = 2;
-        const int hi_all = N - (w0 + pcf::kWave - 1);
-        if (j + 15 <= hi_all) {
-            double c[16];
-#pragma unroll
-            for (int t = 0; t < 16; ++t) c[t] = tab[j + t];
-            for (; j + 31 <= hi_all; j += 16) {
-                double nx[16];  // the next 16 values load while these 16 are subtracted
-#pragma unroll
-                for (int t = 0; t < 16; ++t) nx[t] = tab[j + 16 + t];
-#pragma unroll
-                for (int t = 0; t < 16; ++t) acc -= c[t];
-#pragma unroll
-                for (int t = 0; t < 16; ++t) c[t] = nx[t];
-            }
-#pragma unroll
-            for (int t = 0; t < 16; ++t) acc -= c[t];
-            j += 16;
-        }
-        for (; j <= hi_all; ++j) acc -= tab[j];
-        const int hi_any = N - w0;
-        for (; j <= hi_any; ++j) {
-            const double t = tab[j];
-            acc = (j <= N - i) ? acc - t : acc;
         }
     }
     double comb_val = acc;
@@ -250,15 +221,10 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
-    {
-        // few waves per SIMD (at most 2): the chain's latency bounds the time
-        const int64_t waves = (a.lanes + pcf::kWave - 1) / pcf::kWave;
-        const bool staged = waves <= 2LL * 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
-        auto kern = staged ? k_binom_chain<true> : k_binom_chain<false>;
-        kern<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
-                                          ctx->partials.as<pcf::CSum>(), ctx->counters.as<unsigned int>(),
-                                          ctx->out.as<double>());
-    }
+        k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
+                                                  ctx->partials.as<pcf::CSum>(),
+                                                  ctx->counters.as<unsigned int>(),
+                                                  ctx->out.as<double>());
     else
         k_binom_lgamma<<<grid, B, 0, ctx->stream>>>(a, ctx->partials.as<pcf::CSum>(),
                                                    ctx->counters.as<unsigned int>(),
