@@ -217,3 +217,23 @@ def test_binom_served(engine):
         r = srv.wait(t)
     want = O.lib().orc_binom_embar(100, 100, 0.05, 0.2, 1.0, 10000, 1.0, 1)
     assert abs(r.price - want) / want < 1e-12
+
+
+def test_kept_path_stores_bounded_by_the_budget(engine):
+    """Slots keep their American path stores between requests only while the
+    stores kept by all slots fit the budget; past it a slot frees its store.
+    Results do not depend on which happened: each equals its lone call bit
+    for bit, in two rounds of the same requests."""
+    # An FP64 2^16 x 50 store is 2^16 * 50 * 8 B = 26 MB: with 64 MB and 4 slots
+    # at most two stay kept, so later requests free theirs.
+    N, M = 1 << 16, 50
+    seeds = [11, 12, 13, 14, 15, 16, 17, 18]
+    want = {s: engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision="fp64").price
+            for s in seeds}
+    say("kept path stores, 64 MB budget, 4 slots")
+    with Server(engine, mem_budget_bytes=64 << 20, max_inflight=4) as srv:
+        for _ in range(2):
+            ts = {s: srv.submit("amer", 100, 100, 0.06, 0.2, 1.0, N, "put", M=M, seed=s, precision="fp64")
+                  for s in seeds}
+            for s, t in ts.items():
+                assert srv.wait(t).price == want[s], s
