@@ -229,6 +229,147 @@ double orc_binom_embar_prefix(double S0, double E, double r, double sigma, doubl
     return exp(-r * T) * result;
 }
 
+// Production binomial sum: the same beta tree and terms as orc_binom_embar,
+// with each weight C(N,i) p^i q^(N-i) formed in log space in double-double
+// arithmetic, so the weights carry no rounding of the large intermediate
+// logs (log N! ~ 1.3e7 at N = 1e6, whose FP64 ulp is 1.9e-9).
+//   lf[k] = log k! : double-double table (orc_logfact_dd);
+//   lp, lq         : log p, log q as double-double (dd_log, no libm);
+//   t_i = lf[N] - lf[i] - lf[N-i] + i lp + (N-i) lq   (double-double),
+//   w_i = e + e t_i.lo with e = exp(t_i.hi);
+//   term_i = w_i payoff(S0 pow(u,i) pow(d,N-i))       (terms with w_i = 0 skipped)
+// and exp(-rT) times the sum.  The double-double steps are error-free
+// transformations (TwoSum; Dekker's split for TwoProduct, exact without FMA),
+// so a device restatement with FMA forms the same t_i bit for bit.
+// Pinned: the exact sum (mpmath, 40 digits, the same double u, d, p, q) at
+// N = 10^3, 10^5, 10^6, call and put, to 1e-13 (tests/golden/binom_exact.csv).
+// The reference's own comb() chain is 1.2e-10 (N = 1e5) and 1.5e-8 (N = 1e6)
+// away from that exact sum.
+struct DD {
+    double h, l;
+};
+static DD dd_two_sum(double a, double b) {
+    const double s = a + b, bp = s - a;
+    return {s, (a - (s - bp)) + (b - bp)};
+}
+static DD dd_quick(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+static void dd_split(double a, double& hi, double& lo) {
+    const double c = 134217729.0 * a;  // 2^27 + 1
+    hi = c - (c - a);
+    lo = a - hi;
+}
+static DD dd_two_prod(double a, double b) {
+    const double p = a * b;
+    double ah, al, bh, bl;
+    dd_split(a, ah, al);
+    dd_split(b, bh, bl);
+    return {p, ((ah * bh - p) + ah * bl + al * bh) + al * bl};
+}
+static DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.h, b.h);
+    s.l += a.l + b.l;
+    return dd_quick(s.h, s.l);
+}
+static DD dd_mul_d(DD a, double b) {
+    DD p = dd_two_prod(a.h, b);
+    p.l += a.l * b;
+    return dd_quick(p.h, p.l);
+}
+
+static DD dd_mul(DD a, DD b) {
+    DD p = dd_two_prod(a.h, b.h);
+    p.l += a.h * b.l + a.l * b.h;
+    return dd_quick(p.h, p.l);
+}
+// a / b for a double-double a and a double b: q = a.h / b, then the exact
+// remainder a - q b (TwoProduct) divided once more.
+static DD dd_div_d(DD a, double b) {
+    const double q = a.h / b;
+    const DD qb = dd_two_prod(q, b);
+    const double rem = ((a.h - qb.h) - qb.l) + a.l;
+    return dd_quick(q, rem / b);
+}
+
+static DD dd_div(DD a, DD b) {
+    const double q1 = a.h / b.h;
+    const DD r = dd_add(a, dd_mul_d(DD{-b.h, -b.l}, q1));
+    const double q2 = r.h / b.h;
+    return dd_quick(q1, q2);
+}
+// 2 atanh(t) = sum 2 t^(2j+1)/(2j+1), summed until a term is below 1e-34 of the sum.
+static DD dd_2atanh(DD t) {
+    const DD u = dd_mul(t, t);
+    DD sum = t, pw = t;
+    for (int j = 1; j < 200; ++j) {
+        pw = dd_mul(pw, u);
+        const DD term = dd_div_d(pw, (double)(2 * j + 1));
+        sum = dd_add(sum, term);
+        if (fabs(term.h) < 1e-34 * fabs(sum.h)) break;
+    }
+    return DD{2.0 * sum.h, 2.0 * sum.l};
+}
+static const DD kLn2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
+// log x (x > 0 finite) as double-double without libm: x = m 2^e with m in
+// [0.75, 1.5), log x = e log 2 + 2 atanh((m - 1)/(m + 1)).
+static DD dd_log(double x) {
+    int e = 0;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.75) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const DD t = dd_div(DD{m - 1.0, 0.0}, dd_two_sum(m, 1.0));
+    return dd_add(dd_mul_d(kLn2, (double)e), dd_2atanh(t));
+}
+
+// lf[k] = log k!, k = 0..N, as double-double (hi, lo interleaved: out[2k],
+// out[2k+1]), from IEEE double operations only (no libm), so the table is the
+// same on every host:
+//   log 2 = the double-double constant (0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56);
+//   log k = log(k-1) + 2 atanh(1/(2k-1))  (k >= 3), atanh(t) = sum t^(2j+1)/(2j+1)
+//           summed until a term falls below 1e-34 of the sum;
+//   lf[k] = lf[k-1] + log k.
+// Each step rounds at ~1e-32 relative, so after 10^6 steps lf[N] ~ 1.3e7 is
+// within ~1e-20 of log N!.
+void orc_logfact_dd(int N, double* out) {
+    out[0] = out[1] = 0;
+    if (N >= 1) out[2] = out[3] = 0;
+    DD Lk = kLn2, lf = {0.0, 0.0};
+    for (int k = 2; k <= N; ++k) {
+        if (k > 2) Lk = dd_add(Lk, dd_2atanh(dd_div_d(DD{1.0, 0.0}, (double)(2 * k - 1))));
+        lf = dd_add(lf, Lk);
+        out[2 * k] = lf.h;
+        out[2 * k + 1] = lf.l;
+    }
+}
+
+double orc_binom_dd(double S0, double E, double r, double sigma, double T, int N, double payoff_fun) {
+    const int pf = (int)payoff_fun;
+    double tree[4];
+    orc_beta_tree(r, sigma, T, N, tree);
+    const double u = tree[0], d = tree[1], p = tree[2];
+    const double q = 1 - p;
+    std::vector<double> lf(2 * ((size_t)N + 1));
+    orc_logfact_dd(N, lf.data());
+    const DD lp = dd_log(p), lq = dd_log(q);
+    const DD lfN = {lf[2 * (size_t)N], lf[2 * (size_t)N + 1]};
+    long double acc = 0;
+    for (int i = 0; i <= N; ++i) {
+        DD t = dd_add(lfN, DD{-lf[2 * (size_t)i], -lf[2 * (size_t)i + 1]});
+        t = dd_add(t, DD{-lf[2 * (size_t)(N - i)], -lf[2 * (size_t)(N - i) + 1]});
+        t = dd_add(t, dd_mul_d(lp, (double)i));
+        t = dd_add(t, dd_mul_d(lq, (double)(N - i)));
+        const double e = exp(t.h);
+        const double w = e + e * t.l;
+        if (w == 0.0) continue;
+        acc += (long double)(w * payoff(S0 * pow(u, i) * pow(d, N - i), E, pf));
+    }
+    return exp(-r * T) * (double)acc;
+}
+
 // Vanilla lattices -- src/binom_vanilla_eur.cpp:5-39 and
 // src/binom_vanilla_amer.cpp:5-40 (backward induction on the same beta tree).
 // Pinned: reference driver (17 digits); the European one also against the
@@ -557,6 +698,49 @@ double orc_basket_z(double S0, double E, double r, double sigma, double T,
     return (exp(-r * T) * result) / (double)N;
 }
 
+// Production basket (FP64): the reference's estimator with the sqrt(T) that
+// mvnorm leaves out (include/mvn.h:78-80 samples L Z with Z ~ N(0, I) and
+// src/mc_eur_multi.cpp:30 multiplies by sigma only), i.e. the terminal value
+// of each asset of a GBM basket:
+//     S_a = S0_a exp((r - s_a^2/2) T + (s_a sqrt(T)) B_a),  B = L z.
+// Z column-major d x N (asset fastest) as in orc_basket_z; sig/s0/wt may be
+// NULL (scalar sigma, S0, weights 1/d).  se_out (optional) receives the
+// standard error.
+// Pinned: d = 1 equals orc_mc_eur_z on the draws z sqrt(T) (the reference's
+// mc_eur, src/mc_eur.cpp:15-27); a rank-one factor (rho = 1) prices the
+// single-asset Black-Scholes value within a few standard errors at T != 1.
+double orc_basket_prod_z(double S0, double E, double r, double sigma, double T, int64_t N, double payoff_fun,
+                         int d, const double* L, const double* Z, const double* sig, const double* s0,
+                         const double* wt, double* se_out) {
+    const int pf = (int)payoff_fun;
+    const double sqT = sqrt(T);
+    std::vector<double> c0(d), c1(d), ws(d), B(d);
+    for (int a = 0; a < d; ++a) {
+        const double sg = sig ? sig[a] : sigma;
+        c0[a] = (r - pow(sg, 2) / 2) * T;
+        c1[a] = sg * sqT;
+        ws[a] = (wt ? wt[a] : 1.0 / (double)d) * (s0 ? s0[a] : S0);
+    }
+    double s = 0, s2 = 0;
+    for (int64_t n = 0; n < N; ++n) {
+        const double* z = Z + n * (int64_t)d;
+        double basket = 0;
+        for (int a = 0; a < d; ++a) {
+            double acc = 0;
+            for (int k = 0; k < d; ++k) acc += L[a * d + k] * z[k];
+            basket += ws[a] * exp(c0[a] + c1[a] * acc);
+        }
+        const double v = payoff(basket, E, pf);
+        s += v;
+        s2 += v * v;
+    }
+    const double disc = exp(-r * T);
+    const double mean = s / N;
+    const double var = (s2 - s * mean) / (double)(N > 1 ? N - 1 : 1);
+    if (se_out) *se_out = disc * sqrt(var / N);
+    return (disc * s) / (double)N;
+}
+
 // Lower Cholesky factor of a d x d SPD matrix (row-major), FP64, textbook
 // column-by-column order.  Returns 0 if not positive definite.
 // Pinned: L L^T == C residual and the equicorrelation closed form in tests.
@@ -655,31 +839,87 @@ double orc_philox_mc_eur(double S0, double E, double r, double sigma, double T,
     return disc * mean;
 }
 
-// mc_asia production estimator (FP64): the reference's recurrence
-// (src/mc_asia.cpp:31-35) driven by the Philox stream, step m of path p uses
-// normal m.  geo_out (optional) receives the discounted price of the
-// geometric-average control on the same paths (average of log S at
-// t_1..t_M), whose closed form is orc_geo_asian().
+// mc_asia production arithmetic (FP64): the recurrence of src/mc_asia.cpp:31-35
+// written on standard normals z,
+//     I += S (a1 + a2 z),  S *= exp(mu + vol z),
+//     a1 = 1 + r dt/2, a2 = sigma sqrt(dt)/2, mu = (r - sigma^2/2) dt, vol = sigma sqrt(dt),
+// which the reference writes on its draws dB = sqrt(dt) z.  The geometric
+// control averages log S over t_1..t_M on the same path.
+struct AsiaProd {
+    double a1, a2, mu, vol, sdt;
+};
+static AsiaProd asia_prod(double r, double sigma, double T, int M) {
+    AsiaProd c;
+    const double dt = T / M;
+    c.sdt = sqrt(dt);
+    c.a1 = 1 + r * dt / 2;
+    c.a2 = sigma * c.sdt / 2;
+    c.mu = (r - 0.5 * sigma * sigma) * dt;
+    c.vol = sigma * c.sdt;
+    return c;
+}
+extern "C++" {
+template <class ZF>
+static void asia_prod_path(const AsiaProd& c, double S0, int M, ZF z_of, double* avg, double* geo_log) {
+    double S = S0, I = 0, ls = log(S0), lsum = 0;
+    for (int m = 0; m < M; ++m) {
+        const double z = z_of(m);
+        I += S * (c.a1 + c.a2 * z);
+        S *= exp(c.mu + c.vol * z);
+        ls += c.mu + c.vol * z;
+        lsum += ls;
+    }
+    *avg = I / M;
+    *geo_log = lsum / M;
+}
+}  // extern "C++"
+
+// The production arithmetic fed by the reference's draws: w[n*M+m] are
+// N(0, sqrt(dt)) as in orc_mc_asia_z, and z = w / sqrt(dt).  This is the
+// production form on the reference's own stream, so it must agree with
+// orc_mc_asia_z to rounding: a wrong a1, a2, mu or vol does not.
+// Pinned: orc_mc_asia_z on the same draws to 1e-12 (tests/test_oracle.py).
+double orc_mc_asia_prod_z(double S0, double E, double r, double sigma, double T, int64_t N, int M,
+                          double payoff_fun, const double* w, double* se_out) {
+    const int pf = (int)payoff_fun;
+    const AsiaProd c = asia_prod(r, sigma, T, M);
+    double s = 0, s2 = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        const double* wp = w + p * (int64_t)M;
+        double avg, gl;
+        asia_prod_path(c, S0, M, [&](int m) { return wp[m] / c.sdt; }, &avg, &gl);
+        const double v = payoff(avg, E, pf);
+        s += v;
+        s2 += v * v;
+    }
+    const double disc = exp(-r * T);
+    const double mean = s / N;
+    const double var = (s2 - s * mean) / (double)(N > 1 ? N - 1 : 1);
+    if (se_out) *se_out = disc * sqrt(var / N);
+    return disc * mean;
+}
+
+// mc_asia production estimator (FP64): the production arithmetic above driven
+// by the Philox stream, step m of path p uses normal m.  geo_out (optional)
+// receives the discounted price of the geometric-average control on the same
+// paths, whose closed form is orc_geo_asian().
+// Pinned: the arithmetic through orc_mc_asia_prod_z; the normals through
+// orc_philox_normals4's known answers; the estimator statistically against
+// orc_geo_asian.
 double orc_philox_mc_asia(double S0, double E, double r, double sigma, double T,
                           int64_t N, int M, double payoff_fun, uint64_t seed,
                           double* se_out, double* geo_out) {
     const int pf = (int)payoff_fun;
-    double dt = T / M, sdt = sqrt(dt);
-    double a1 = 1 + r * dt / 2, a2 = sigma * sdt / 2;
-    double mu = (r - 0.5 * sigma * sigma) * dt, vol = sigma * sdt;
+    const AsiaProd c = asia_prod(r, sigma, T, M);
     double s = 0, s2 = 0, g = 0;
     for (int64_t p = 0; p < N; ++p) {
-        double S = S0, I = 0, lsum = 0, ls = log(S0);
-        for (int m = 0; m < M; ++m) {
-            double z = pnorm(seed, ST_ASIA, (uint64_t)p, (uint32_t)m);
-            I += S * (a1 + a2 * z);
-            S *= exp(mu + vol * z);
-            ls += mu + vol * z;
-            lsum += ls;
-        }
-        double v = payoff(I / M, E, pf);
-        s += v; s2 += v * v;
-        g += payoff(exp(lsum / M), E, pf);
+        double avg, gl;
+        asia_prod_path(c, S0, M, [&](int m) { return pnorm(seed, ST_ASIA, (uint64_t)p, (uint32_t)m); }, &avg,
+                       &gl);
+        const double v = payoff(avg, E, pf);
+        s += v;
+        s2 += v * v;
+        g += payoff(exp(gl), E, pf);
     }
     double disc = exp(-r * T);
     double mean = s / N;
@@ -737,44 +977,6 @@ double orc_geo_asian(double S0, double E, double r, double sigma, double T, int 
     double disc = exp(-r * T);
     if (payoff_fun > 0) return disc * (fwd * ncdf(d1) - E * ncdf(d2));
     return disc * (E * ncdf(-d2) - fwd * ncdf(-d1));
-}
-
-// ---------------------------------------------------------------------------
-// OpenMP timing restatement of src/mc_asia_omp.cpp:16-48 (per-thread
-// mt19937 seeded seed*(1+tid), schedule(dynamic,1000)).  CPU baseline only;
-// its price is statistical (streams differ per thread count).
-double orc_mc_asia_omp(double S0, double E, double r, double sigma, double T,
-                       int64_t N, int M, double payoff_fun, int threads, uint64_t seed) {
-    const int pf = (int)payoff_fun;
-    double result = 0;
-    (void)threads;
-#ifdef _OPENMP
-#pragma omp parallel num_threads(threads > 0 ? threads : 1)
-#endif
-    {
-        double dt = (double)T / (double)M;
-        int thr_num = 0;
-#ifdef _OPENMP
-        thr_num = omp_get_thread_num();
-#endif
-        std::mt19937 gen;
-        gen.seed(seed * (1 + thr_num));
-        std::normal_distribution<> norm{0, sqrt(dt)};
-#ifdef _OPENMP
-#pragma omp for nowait schedule(dynamic, 1000) reduction(+ : result)
-#endif
-        for (int64_t n = 0; n < N; ++n) {
-            double St = S0;
-            double I = 0;
-            for (int m = 0; m < M; ++m) {
-                double dBi = norm(gen);
-                I += St * (1 + r * dt / 2 + sigma * dBi / 2);
-                St *= exp((r - pow(sigma, 2) / 2) * dt + sigma * dBi);
-            }
-            result += payoff(I / (double)M, E, pf);
-        }
-    }
-    return (exp(-r * T) * result) / (double)N;
 }
 
 int orc_omp_max_threads(void) {

@@ -18,9 +18,13 @@ _SIG = {
     "orc_mc_eur_seed": (_d, [_d, _d, _d, _d, _d, _i64, _d, _u64]),
     "orc_mc_asia_z": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd]),
     "orc_mc_asia_seed": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _u64]),
+    "orc_mc_asia_prod_z": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd, _pd]),
     "orc_comb": (_d, [_i32, _i32]),
     "orc_beta_tree": (None, [_d, _d, _d, _i32, _pd]),
     "orc_binom_embar": (_d, [_d, _d, _d, _d, _d, _i32, _d, _i32]),
+    "orc_binom_embar_prefix": (_d, [_d, _d, _d, _d, _d, _i32, _d, _i32]),
+    "orc_logfact_dd": (None, [_i32, _pd]),
+    "orc_binom_dd": (_d, [_d, _d, _d, _d, _d, _i32, _d]),
     "orc_binom_vanilla": (_d, [_d, _d, _d, _d, _d, _i32, _d, _i32]),
     "orc_inverse3_solve": (_i32, [_pd, _pd, _pd]),
     "orc_mc_amer_z": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd, _i32, C.POINTER(C.c_int32)]),
@@ -29,6 +33,7 @@ _SIG = {
     "orc_mc_amer_poly": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _pd, _i32, C.POINTER(C.c_int32),
                               C.POINTER(C.c_int32)]),
     "orc_basket_z": (_d, [_d, _d, _d, _d, _d, _i64, _d, _i32, _pd, _pd, _pd, _pd, _pd]),
+    "orc_basket_prod_z": (_d, [_d, _d, _d, _d, _d, _i64, _d, _i32, _pd, _pd, _pd, _pd, _pd, _pd]),
     "orc_cholesky": (_i32, [_i32, _pd, _pd]),
     "orc_philox4x32_10": (None, [C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     "orc_philox_normals4": (None, [_u64, _u32, _u64, _u32, _pd]),
@@ -39,7 +44,6 @@ _SIG = {
     "orc_philox_normals": (None, [_u64, _u32, _u64, _i64, _i32, _pd]),
     "orc_bs_price": (_d, [_d, _d, _d, _d, _d, _d]),
     "orc_geo_asian": (_d, [_d, _d, _d, _d, _d, _i32, _d]),
-    "orc_mc_asia_omp": (_d, [_d, _d, _d, _d, _d, _i64, _i32, _d, _i32, _u64]),
     "orc_omp_max_threads": (_i32, []),
 }
 

@@ -414,3 +414,135 @@ def test_philox_production_estimators_statistics():
     a = L.orc_philox_mc_asia(100, 100, 0.05, 0.25, 1.0, 4096, 252, 1.0, 3, C.byref(se), C.byref(geo))
     assert a > geo.value  # arithmetic average dominates the geometric one path by path
     assert abs(geo.value - 6.551716578) < 0.5
+
+
+# ---------------------------------------------------------------------------
+# Binomial: the prefix form, the double-double production form.
+
+PREFIX_N = [1, 2, 3, 100, 101, 1000, 4097, 10000, 30000]
+
+
+@pytest.mark.parametrize("N", PREFIX_N)
+@pytest.mark.parametrize("pf", [1.0, -1.0], ids=["call", "put"])
+def test_binom_prefix_equals_chain_bitwise(N, pf):
+    """orc_binom_embar_prefix (one shared descending prefix, the second loop
+    replayed per term) equals the reference restatement orc_binom_embar
+    bit for bit, so the N = 1e5 / 1e6 fixtures it writes are the reference's
+    values (src/binom_embar.cpp:15-50, include/common.h:63-72)."""
+    L = O.lib()
+    a = L.orc_binom_embar(100, 100, 0.05, 0.2, 1.0, N, pf, 1)
+    b = L.orc_binom_embar_prefix(100, 100, 0.05, 0.2, 1.0, N, pf, 4)
+    assert a == b, (a, b)
+
+
+def test_binom_prefix_matches_reference_driver_at_1e5():
+    """The reference's own kernel at N = 1e5 (ref_kernels.csv, written by the
+    driver built from src/binom_embar.cpp) equals the prefix form, and so
+    does binom_large.csv's N = 1e5 call row."""
+    row = [r for r in _golden_rows() if r["prog"] == "binom_embar" and r["N"] == "100000"][0]
+    got = O.lib().orc_binom_embar_prefix(100, 100, 0.05, 0.2, 1.0, 100000, 1.0, os.cpu_count() or 1)
+    assert got == float(row["result"]), (got, row["result"])
+    with open(os.path.join(GOLDEN, "binom_large.csv")) as f:
+        large = {(r["payoff"], int(r["N"])): float(r["result"]) for r in csv.DictReader(f)}
+    assert large[("call", 100000)] == got
+
+
+def _exact_rows():
+    with open(os.path.join(GOLDEN, "binom_exact.csv")) as f:
+        return list(csv.DictReader(f))
+
+
+@pytest.mark.parametrize("row", _exact_rows(), ids=lambda r: f"{r['payoff']}-N{r['N']}-E{r['E']}")
+def test_binom_dd_matches_the_exact_sum(row):
+    """The double-double production form is within 1e-13 of the exact sum
+    (mpmath, 40 digits, same double tree parameters)."""
+    pf = 1.0 if row["payoff"] == "call" else -1.0
+    S0, E, r, s, T = (float(row[k]) for k in ("S0", "E", "r", "sigma", "T"))
+    N = int(row["N"])
+    exact = float(row["exact"])
+    got = O.lib().orc_binom_dd(S0, E, r, s, T, N, pf)
+    assert abs(got - exact) <= 1e-13 * exact, (got, exact, (got - exact) / exact)
+
+
+def test_binom_chain_error_against_the_exact_sum():
+    """The reference's comb() chain carries the rounding of its running log
+    sums (up to ~7e5 at N = 1e5, ~6e6 at 1e6): its price is 1.2e-10 (1e5)
+    and 1.5e-8 (1e6) from the exact sum.  This is why the production form is
+    checked against the exact sum, and the reference-exact form against the
+    reference."""
+    exact = {(r["payoff"], int(r["N"])): float(r["exact"]) for r in _exact_rows() if r["E"] == "100.0"}
+    with open(os.path.join(GOLDEN, "binom_large.csv")) as f:
+        chain = {(r["payoff"], int(r["N"])): float(r["result"]) for r in csv.DictReader(f)}
+    for key, lo, hi in ((("call", 100000), 5e-11, 5e-10), (("call", 1000000), 5e-9, 5e-8),
+                        (("put", 1000000), 5e-9, 5e-8)):
+        rel = abs(chain[key] - exact[key]) / exact[key]
+        assert lo < rel < hi, (key, rel)
+
+
+def test_logfact_dd_table():
+    """log k! as double-double: exact small factorials, and mpmath's loggamma
+    at large k to 1e-18 absolute (the FP64 ulp of log 200000! is 2.3e-10)."""
+    import mpmath as mp
+    N = 200000
+    t = np.empty(2 * (N + 1))
+    O.lib().orc_logfact_dd(N, O.ptr(t))
+    for k in (0, 1, 2, 3, 10, 20):
+        assert t[2 * k] + t[2 * k + 1] == pytest.approx(math.log(math.factorial(k)), abs=1e-15)
+    mp.mp.dps = 40
+    for k in (1000, 65537, N):
+        ex = mp.loggamma(k + 1)
+        err = (mp.mpf(t[2 * k]) + mp.mpf(t[2 * k + 1])) - ex
+        assert abs(float(err)) < 1e-18, (k, float(err))
+        assert abs(t[2 * k + 1]) <= 0.5 * math.ulp(t[2 * k])
+
+
+# ---------------------------------------------------------------------------
+# Production Asian arithmetic and the production basket's sqrt(T).
+
+@pytest.mark.parametrize("S0,E,r,s,T,M,pf", [(100, 100, 0.05, 0.25, 1.0, 252, 1.0),
+                                             (100, 100, 0.05, 0.25, 1.0, 252, -1.0),
+                                             (100, 110, 0.02, 0.75, 2.0, 50, 1.0),
+                                             (90, 100, 0.08, 0.3, 0.5, 7, -1.0)])
+def test_asia_production_arithmetic_matches_reference(S0, E, r, s, T, M, pf):
+    """The production step (a1 + a2 z, exp(mu + vol z) on z = dB / sqrt(dt))
+    prices the reference's own draws as the reference does
+    (src/mc_asia.cpp:27-37), to rounding."""
+    L = O.lib()
+    N = 4096
+    w = O.normals_mt(17, math.sqrt(T / M), N * M)
+    ref = L.orc_mc_asia_z(S0, E, r, s, T, N, M, pf, O.ptr(w))
+    prod = L.orc_mc_asia_prod_z(S0, E, r, s, T, N, M, pf, O.ptr(w), None)
+    assert abs(prod - ref) <= 1e-12 * abs(ref), (prod, ref)
+
+
+@pytest.mark.parametrize("T", [0.5, 2.0])
+def test_basket_production_applies_sqrt_T(T):
+    """d = 1: the production basket equals the reference's mc_eur on the
+    draws z sqrt(T) (src/mc_eur.cpp:15-27); rank-one factor (rho = 1, four
+    equal assets): the single-asset Black-Scholes price within 4 SE."""
+    L = O.lib()
+    N = 1 << 16
+    z = O.normals_mt(5, 1.0, N)
+    one = np.ones(1)
+    b = L.orc_basket_prod_z(100, 100, 0.05, 0.2, T, N, 1.0, 1, O.ptr(one), O.ptr(z), None, None, None, None)
+    e = L.orc_mc_eur_z(100, 100, 0.05, 0.2, T, N, 1.0, O.ptr(z * math.sqrt(T)))
+    assert abs(b - e) <= 1e-13 * e, (b, e)
+    # the reference form (no sqrt(T)) differs at T != 1
+    bref = L.orc_basket_z(100, 100, 0.05, 0.2, T, N, 1.0, 1, O.ptr(one), O.ptr(z), None, None, None)
+    assert abs(bref - e) > 1e-3
+    d = 4
+    Lf = np.zeros((d, d))
+    Lf[:, 0] = 1.0
+    M = 1 << 17
+    Z = np.empty(M * d)
+    L.orc_philox_basket_normals(11, M, d, O.ptr(Z))
+    se = C_double()
+    for pf in (1.0, -1.0):
+        v = L.orc_basket_prod_z(100, 100, 0.05, 0.2, T, M, pf, d, O.ptr(Lf), O.ptr(Z), None, None, None,
+                                O.C.byref(se))
+        bs = L.orc_bs_price(100, 100, 0.05, 0.2, T, pf)
+        assert abs(v - bs) < 4 * se.value, (T, pf, v, bs, se.value)
+
+
+def C_double():
+    return O.C.c_double()
