@@ -172,15 +172,25 @@ def per_config(eng):
     bs = 10.450583572185565
     r32, r64 = pair(lambda p: eng.mc_eur(100, 100, 0.05, 0.2, 1.0, 10**6, "call", seed=1, precision=p))
     summary("mc_eur_1e6", r32, r64, 1, {"black_scholes": bs, "fp64_minus_bs_in_se": (r64.price - bs) / r64.std_err})
+    # exact sums (mpmath, 40 digits) for the production form's accuracy
+    import csv
+    with open(os.path.join(ROOT, "tests", "golden", "binom_exact.csv")) as f:
+        exact = {int(r["N"]): float(r["exact"]) for r in csv.DictReader(f)
+                 if r["payoff"] == "call" and r["E"] == "100.0"}
     for N in (10**5, 10**6):
         eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
+        eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
         rc = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
-        rl = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
-        out[f"binom_embar_N{N}"] = {"price_chain": rc.price, "kernel_ms_chain": rc.kernel_ms,
-                                         "price_lgamma": rl.price, "kernel_ms_lgamma": rl.kernel_ms,
-                                         "lgamma_minus_chain_rel": (rl.price - rc.price) / rc.price,
-                                         "log_adds_per_s_chain": rc.flops / (rc.kernel_ms * 1e-3)}
-        log(f"binom_embar N={N}", out[f"binom_embar_N{N}"])
+        rd = eng.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
+        d = {"price_chain": rc.price, "kernel_ms_chain": rc.kernel_ms,
+             "price_dd": rd.price, "kernel_ms_dd": rd.kernel_ms,
+             "log_adds_per_s_chain": rc.flops / (rc.kernel_ms * 1e-3)}
+        if N in exact:
+            d["exact"] = exact[N]
+            d["dd_minus_exact_rel"] = (rd.price - exact[N]) / exact[N]
+            d["chain_minus_exact_rel"] = (rc.price - exact[N]) / exact[N]
+        out[f"binom_embar_N{N}"] = d
+        log(f"binom_embar N={N}", d)
     for d in (8, 32):
         sig = np.random.default_rng(d).uniform(0.15, 0.35, size=d)
         r32, r64 = pair(lambda p: eng.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, 1 << 24, "call", d, 0.3,
@@ -203,6 +213,7 @@ def per_config(eng):
     # sizes its runscripts use for the comparison column (runscript_mc_eur.sh:22,
     # runscript_mc_amer.sh:23), each checked against the reference's own value.
     for name, N, pf, r_, am, want in (("lattice_eur_call_N1e5", 100000, "call", 0.05, False, None),
+                                      ("lattice_eur_call_N1e4", 10000, "call", 0.05, False, None),
                                       ("lattice_amer_put_N1e4", 10000, "put", 0.06, True, 5.7989254457596333),
                                       ("lattice_amer_put_N2000", 2000, "put", 0.06, True, 5.798877611783797)):
         eng.binom_vanilla(100, 100, r_, 0.2, 1.0, N, pf, american=am)
@@ -237,8 +248,9 @@ def asia_roofline(eng, K=10):
     kms = sum(r.kernel_ms for r in rs) / K
     fl = rs[0].flops
     ach = fl / (kms * 1e-3) / 1e12
-    return {"bound": "mfma",
-            "pipe": "VALU FP32 (non-packed FMA + transcendental units); 157.3 TFLOP/s is MI355X's FP32 peak",
+    return {"bound": "valu",
+            "pipe": "VALU FP32 (non-packed FMA, 32x32->64 integer multiplies of Philox, transcendentals); no MFMA; "
+                    "157.3 TFLOP/s is MI355X's FP32 vector peak",
             "kernel": "k_asia<fp32k> (mc_asia, 2^22 paths x 252 steps)", "achieved": ach,
             "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": traffic_of("k_asia<0>"),
             "kernel_ms": kms, "flops_per_launch": fl,
@@ -252,10 +264,15 @@ def lsm_roofline(eng, K=10):
     kms = sum(r.kernel_ms for r in rs) / K
     by = rs[0].bytes
     ach = by / (kms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": f"k_lsm_persistent<float> (mc_amer put, 2^20 paths x 50 dates, grid "
-                                      f"{rs[0].grid_blocks}x{rs[0].block_size})",
+    traffic = traffic_of("k_lsm_persistent<float>")
+    return {"bound": "latency",
+            "limiter": "the per-date grid all-reduce (publish, wait for every block, gather, solve, decide): "
+                       "~6 us per date at 2^20 paths against ~0.5 us of row traffic",
+            "kernel": f"k_lsm_persistent<float> (mc_amer put, 2^20 paths x 50 dates, grid "
+                      f"{rs[0].grid_blocks}x{rs[0].block_size})",
             "achieved": ach, "peak": PEAK["hbm"], "unit": "GB/s", "frac": ach / PEAK["hbm"],
-            "traffic": traffic_of("k_lsm_persistent<float>"), "kernel_ms": kms, "bytes_per_launch": by,
+            "frac_of_hbm_from_counters": (traffic / (kms * 1e-3) / 1e9 / PEAK["hbm"]) if traffic else None,
+            "traffic": traffic, "kernel_ms": kms, "bytes_per_launch": by,
             "bytes_per_unit": "24 per path-date in FP32: store write 4 + backward read 4 + per-path state "
                               "{cash flow 4, date 4} read and written (the state is register-resident here, "
                               "so traffic is below this count)"}
@@ -270,7 +287,9 @@ def basket_roofline(eng, K=10):
     kms = sum(r.kernel_ms for r in rs) / K
     fl = rs[0].flops
     ach = fl / (kms * 1e-3) / 1e12
-    return {"bound": "mfma", "kernel": "k_basket_mfma (mc_eur_multi d=32, 2^24 paths)", "achieved": ach,
+    return {"bound": "valu", "pipe": "Z L^T on MFMA (v_mfma_f32_32x32x2f32); issue-bound on the VALU by the "
+                                     "Philox normals and the per-asset exp",
+            "kernel": "k_basket_mfma (mc_eur_multi d=32, 2^24 paths)", "achieved": ach,
             "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
             "traffic": traffic_of("k_basket_mfma"), "kernel_ms": kms, "flops_per_launch": fl,
             "flops_per_unit": "d^2 + 12 d + 3 per path (normals 6d, Z L^T d(d+1), asset 5d, payoff 3)"}
@@ -322,77 +341,120 @@ def profile_one(eng, which, K, W):
 
 
 # ---------------------------------------------------------------- CPU baseline
-def cpu_binomial(ref, env, threads):
-    """One run each of the reference's binomial programs (built from src/ into
-    oracle/_ref), in the same units as the GPU's per-configuration figures:
-    binom_embar_omp at N = 10^5 (reference adds 0.75 N^2 per second),
-    binom_vanilla_amer at N = 10^4 and binom_vanilla_eur at N = 10^4 (node
-    updates (N+1)(N+2)/2 per second; the European lattice is timed at 10^4, not
-    the GPU's 10^5, because the serial program takes minutes there).  About
-    5-10 s of CPU time."""
-    legs = {"binom_embar_N100000": (["binom_embar_omp", "call", 100, 100, 0.05, 0.2, 1.0, 100000, threads],
-                                    lambda N, t: {"log_adds_per_s_chain": 0.75 * N * N / t}),
-            "lattice_amer_put_N1e4": (["binom_vanilla_amer", "put", 100, 100, 0.06, 0.2, 1.0, 10000],
-                                      lambda N, t: {"node_updates_per_s": 0.5 * (N + 1) * (N + 2) / t}),
-            "lattice_eur_call_N1e4": (["binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1.0, 10000],
-                                      lambda N, t: {"node_updates_per_s": 0.5 * (N + 1) * (N + 2) / t})}
-    out = {}
-    for name, (argv, rate) in legs.items():
-        exe = os.path.join(ref, argv[0])
-        if not os.path.exists(exe):
-            out[name] = None
+def cpu_threads():
+    """Threads for the reference's OpenMP programs: every core this process may
+    run on, at most 16 (one GPU's share of the host on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _ref_row(ref, env, argv):
+    exe = os.path.join(ref, argv[0])
+    row = subprocess.run([exe] + [str(x) for x in argv[1:]], env=env, check=True, capture_output=True,
+                         text=True, timeout=900).stdout.strip().split(",")
+    return float(row[12]), float(row[13])  # T_calculation, Result
+
+
+# (name, argv, work units, unit) -- each configuration at its own size.  The
+# work unit is the one the GPU figure of the same configuration uses.
+def cpu_legs(threads, full):
+    legs = [
+        ("mc_eur_1e6", ["mc_eur_omp", "call", 100, 100, 0.05, 0.2, 1.0, 10**6, threads], 10**6, "paths·steps"),
+        ("mc_asia_2^22x252", ["mc_asia_omp", "call", 100, 100, 0.05, 0.25, 1.0, 1 << 22, 252, threads],
+         (1 << 22) * 252, "paths·steps"),
+        ("mc_amer_put_2^20x50", ["mc_amer_omp", "put", 100, 100, 0.06, 0.2, 1.0, 1 << 20, 50, threads],
+         (1 << 20) * 50, "paths·steps"),
+        ("binom_embar_N100000", ["binom_embar_omp", "call", 100, 100, 0.05, 0.2, 1.0, 100000, threads],
+         0.75 * 1e10, "reference log-adds (0.75 N^2)"),
+        ("lattice_amer_put_N1e4", ["binom_vanilla_amer", "put", 100, 100, 0.06, 0.2, 1.0, 10000],
+         0.5 * 10001 * 10002, "node updates"),
+        ("lattice_eur_call_N1e4", ["binom_vanilla_eur", "call", 100, 100, 0.05, 0.2, 1.0, 10000],
+         0.5 * 10001 * 10002, "node updates"),
+    ]
+    if full:
+        legs.append(("binom_embar_N1000000", ["binom_embar_omp", "call", 100, 100, 0.05, 0.2, 1.0, 10**6, threads],
+                     0.75 * 1e12, "reference log-adds (0.75 N^2)"))
+    return legs
+
+
+def cpu_reference(full=False, repeats=3):
+    """The reference's own programs (src/*_omp.cpp at -O2 -fopenmp, the serial
+    lattices as shipped at -O0; built from source into oracle/_ref), each at its
+    configuration's own size: best of `repeats` T_calculation (the O(N^2)
+    binom_embar at N = 1e6, only with full=True, is timed once).  The basket
+    program is not buildable here (Eigen and Boost are absent)."""
+    threads = cpu_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
+    out = {"threads": threads, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+           "compiler": "g++ -O2 -fopenmp (the reference ships -O0, Makefile:9); lattices -O0 as shipped",
+           "legs": {}}
+    for name, argv, work, unit in cpu_legs(threads, full):
+        if not os.path.exists(os.path.join(ref, argv[0])):
+            out["legs"][name] = None
             continue
-        row = subprocess.run([exe] + [str(x) for x in argv[1:]], env=env, check=True, capture_output=True,
-                             text=True, timeout=600).stdout.strip().split(",")
-        t, N = float(row[12]), int(argv[7])
-        out[name] = dict({"seconds": t, "N": N, "price": float(row[13]),
-                          "threads": threads if argv[0].endswith("_omp") else 1}, **rate(N, t))
+        reps = 1 if name == "binom_embar_N1000000" else repeats
+        runs = [_ref_row(ref, env, argv) for _ in range(reps)]
+        t = min(r[0] for r in runs)
+        out["legs"][name] = {"program": argv[0], "argv": " ".join(str(x) for x in argv[1:]),
+                             "seconds_best": t, "runs": reps, "result": runs[0][1], "work": work, "unit": unit,
+                             "per_s": work / t, "threads": threads if argv[0].endswith("_omp") else 1}
+        log("cpu", name, out["legs"][name])
     return out
 
 
-def cpu_baseline(mix, budget_s):
-    threads = max(1, min(16, os.cpu_count() or 1))
-    ref = os.path.join(ROOT, "oracle", "_ref")
-    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
-    exes = {"eur": "mc_eur_omp", "asia": "mc_asia_omp", "amer": "mc_amer_omp"}
-    if not all(os.path.exists(os.path.join(ref, e)) for e in exes.values()):
-        return {"value": None, "unit": "paths·steps/s", "cores": threads, "kind": None,
+def attach_cpu(configs, cpu):
+    """Puts each CPU leg beside the GPU figure of the same configuration and
+    size, with the GPU/CPU ratio in the leg's unit."""
+    gpu_rate = {
+        "mc_eur_1e6": ("paths_steps_per_s_fp32k", "paths_steps_per_s_fp64"),
+        "mc_asia_2^22x252": ("paths_steps_per_s_fp32k", "paths_steps_per_s_fp64"),
+        "mc_amer_put_2^20x50": ("paths_steps_per_s_fp32k", "paths_steps_per_s_fp64"),
+        "binom_embar_N100000": ("log_adds_per_s_chain", None),
+        "binom_embar_N1000000": ("log_adds_per_s_chain", None),
+        "lattice_amer_put_N1e4": ("node_updates_per_s", None),
+        "lattice_eur_call_N1e4": ("node_updates_per_s", None),
+    }
+    for name, leg in cpu["legs"].items():
+        if not leg or name not in configs:
+            continue
+        c = configs[name]
+        c["cpu_reference"] = leg
+        k32, k64 = gpu_rate[name]
+        c["gpu_over_cpu"] = {"gpu_fp32k" if k64 else "gpu_fp64": c[k32] / leg["per_s"]}
+        if k64:
+            c["gpu_over_cpu"]["gpu_fp64"] = c[k64] / leg["per_s"]
+
+
+def cpu_baseline_line(cpu, mix):
+    """The headline CPU figure in the metric's unit: the served mix's
+    paths·steps over the time the reference's per-configuration rates imply
+    for it (European, Asian, American requests; the basket needs Eigen/Boost)."""
+    rate = {}
+    for kind, name in (("eur", "mc_eur_1e6"), ("asia", "mc_asia_2^22x252"), ("amer", "mc_amer_put_2^20x50")):
+        leg = cpu["legs"].get(name)
+        if leg:
+            rate[kind] = leg["per_s"]
+    sample = [q for q in mix if q["kind"] in rate]
+    if not sample:
+        return {"value": None, "unit": "paths·steps/s", "cores": cpu["threads"], "kind": None,
                 "sample": "oracle/_ref reference programs not built"}
-    sample = [q for q in mix if q["kind"] in exes]
-    # scale every request by the same factor so the whole sample is ~budget_s
-    def run(q, N):
-        a = q["args"]
-        argv = [os.path.join(ref, exes[q["kind"]]), a[6]] + [str(x) for x in a[:5]] + [str(N)]
-        if q["kind"] != "eur":
-            argv.append(str(q["kw"]["M"]))
-        argv.append(str(threads))
-        out = subprocess.run(argv, env=env, check=True, capture_output=True, text=True, timeout=600).stdout
-        return float(out.strip().split(",")[12])
-    probe = 1 << 12
-    t_probe = sum(run(q, probe) for q in sample[:8])
-    ps_probe = sum(probe * q["steps"] for q in sample[:8])
-    rate0 = ps_probe / max(t_probe, 1e-6)
-    total_ps = path_steps(sample)
-    scale = min(1.0, budget_s * rate0 / total_ps)
-    tot_t, tot_ps = 0.0, 0
-    kinds = {}
-    for q in sample:
-        N = max(64, int(q["args"][5] * scale) // 2 * 2)
-        t = run(q, N)
-        tot_t += t
-        tot_ps += N * q["steps"]
-        k = kinds.setdefault(q["kind"], [0.0, 0])
-        k[0] += t
-        k[1] += N * q["steps"]
-    by_kind = {k: {"paths_steps_per_s": v[1] / v[0], "paths_steps": v[1], "seconds": v[0]}
-               for k, v in kinds.items()}
-    return {"value": tot_ps / tot_t, "unit": "paths·steps/s", "cores": threads, "kind": "reference",
-            "by_kind": by_kind, "binomial": cpu_binomial(ref, env, threads),
-            "sample": f"{len(sample)} European/Asian/American requests of the served mix with path counts "
-                      f"scaled by {scale:.4f} ({tot_ps:.3g} paths·steps), run by the reference's "
-                      f"mc_eur_omp/mc_asia_omp/mc_amer_omp (built -O2 -fopenmp from src/), FP64, "
-                      f"{threads} threads, OMP_PROC_BIND=close, on '{cpu_model()}'; sum of T_calculation "
-                      f"{tot_t:.2f} s; basket requests excluded (reference needs Eigen/Boost)"}
+    ps = path_steps(sample)
+    t = sum(q["args"][5] * q["steps"] / rate[q["kind"]] for q in sample)
+    total_s = sum(leg["seconds_best"] * leg["runs"] for leg in cpu["legs"].values() if leg)
+    return {"value": ps / t, "unit": "paths·steps/s", "cores": cpu["threads"], "kind": "reference",
+            "nproc": cpu["nproc"], "cpu_model": cpu["cpu_model"],
+            "sample": f"the reference's mc_eur_omp (N=1e6), mc_asia_omp (N=2^22, M=252) and mc_amer_omp "
+                      f"(N=2^20, M=50), each at its configuration's own size, best of 3 T_calculation, "
+                      f"{cpu['threads']} threads (OMP_PROC_BIND=close) on '{cpu['cpu_model']}' "
+                      f"(nproc {cpu['nproc']}); value = the {len(sample)} European/Asian/American requests "
+                      f"of the served mix ({ps:.3g} paths·steps) over the time those rates give "
+                      f"({t:.2f} s); basket requests excluded (the reference basket needs Eigen/Boost); "
+                      f"{total_s:.1f} s of CPU runs in all",
+            "legs": cpu["legs"]}
 
 
 def main():
@@ -403,7 +465,10 @@ def main():
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--latency-requests", type=int, default=400)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the reference's binom_embar_omp at N = 1e6 once (O(N^2): minutes)")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="run only the CPU reference timings and print them as one JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
@@ -416,6 +481,9 @@ def main():
             "collective code. Use --gpus 1.")
         sys.exit(2)
     K, W = args.steps, args.warmup
+    if args.cpu_only:
+        print(json.dumps(cpu_reference(full=args.cpu_full)), flush=True)
+        return
 
     from parcompfin_amd import Engine, Server, autotune
     eng = Engine(0)
@@ -488,17 +556,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(mix, args.cpu_seconds)
+            ref = cpu_reference(full=args.cpu_full)
+            cpu = cpu_baseline_line(ref, mix)
             log("cpu baseline:", cpu)
-            # the CPU's rates beside the GPU's, per configuration
-            for name, kind in (("mc_eur_1e6", "eur"), ("mc_asia_2^22x252", "asia"), ("mc_amer_put_2^20x50", "amer")):
-                if name in configs and kind in cpu.get("by_kind", {}):
-                    configs[name]["cpu_reference_paths_steps_per_s"] = cpu["by_kind"][kind]["paths_steps_per_s"]
-                    configs[name]["cpu_reference_sample"] = "served-mix requests of this kind, scaled; see cpu_baseline"
-            for name, leg in (cpu.get("binomial") or {}).items():
-                key = "lattice_eur_call_N1e5" if name == "lattice_eur_call_N1e4" else name
-                if leg and key in configs:
-                    configs[key]["cpu_reference"] = leg
+            attach_cpu(configs, ref)
         except Exception as e:  # reported, never replaced by something else
             log("cpu baseline failed:", repr(e))
             cpu = {"value": None, "unit": "paths·steps/s", "cores": None, "kind": None, "sample": f"failed: {e!r}"}

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 1
+#define PCF_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum pcf_status {
@@ -91,7 +91,11 @@ typedef struct pcf_opts {
                                 reference's {1, S-E, (S-E)^2} with ref_exact); 2..6 = {1, x, ..,
                                 x^(K-1)}, x = S/E - 1 (production arithmetic; persistent and
                                 graph forms) */
-    int32_t reserved[4];
+    int32_t fp64_check;      /* 1: a PCF_FP32K Philox pricing (mc_eur, mc_asia, mc_amer,
+                                mc_eur_multi, direct or served) is repeated in PCF_FP64 on the
+                                same Philox counters, and pcf_result.fp64_err receives
+                                |price - FP64 price|; the timing fields stay the FP32 run's */
+    int32_t reserved[3];
 } pcf_opts;
 
 typedef struct pcf_result {
@@ -116,7 +120,8 @@ typedef struct pcf_result {
     int32_t grid_blocks;   /* launch shape of the dominant kernel */
     int32_t block_size;
     int32_t batch_size;    /* requests priced by the same launch (1 when alone) */
-    int32_t reserved[3];
+    int32_t fp64_checked;  /* 1 when fp64_err holds the FP64 comparison (opts.fp64_check) */
+    double fp64_err;       /* |price - price of the FP64 form on the same Philox paths| */
 } pcf_result;
 
 typedef struct pcf_ctx pcf_ctx;
@@ -132,7 +137,8 @@ int pcf_device_info(pcf_ctx* ctx, char* name, int name_len, int* compute_units,
 void pcf_opts_init(pcf_opts* opts);
 int pcf_abi_version(void);
 /* sizeof of an ABI record by name ("pcf_opts", "pcf_result", "pcf_basket_ext",
- * "pcf_request", "pcf_server_config", "pcf_request_timing", "pcf_tune_entry");
+ * "pcf_request", "pcf_server_config", "pcf_request_timing", "pcf_tune_entry",
+ * "pcf_server_stats");
  * -1 if unknown.
  * Lets FFI bindings verify their struct mirrors without a device. */
 int pcf_struct_size(const char* name);
@@ -148,13 +154,20 @@ int pcf_mc_eur(pcf_ctx* ctx, double S0, double E, double r, double sigma, double
 /* Closed binomial sum over the beta tree (src/binom_embar.cpp:19-27).
  * Replaces double binom(S0,E,r,sigma,T,N,payoff_fun), src/binom_embar.cpp:5-50.
  * ref_exact = 1 (default through the CLI) replays comb() of include/common.h:63-72
- * bit for bit; ref_exact = 0 uses FP64 lgamma coefficients. */
+ * bit for bit; ref_exact = 0 forms each weight C(N,i) p^i q^(N-i) in log space
+ * in double-double arithmetic (a resident table of log k!), within ~1e-15 of
+ * the exact sum (the reference's chain is 1.2e-10 from it at N = 1e5 and
+ * 1.5e-8 at N = 1e6). */
 int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
                     int64_t N, double payoff_fun, const pcf_opts* opts, pcf_result* res);
 
 /* Fixed-strike arithmetic Asian with the reference's trapezoid running sum.
  * Replaces double mc_asia(S0,E,r,sigma,T,N,M,payoff_fun), src/mc_asia.cpp:5-40.
- * PCF_RNG_HOST: host_normals[n*M+m] are the draws N(0, sqrt(T/M)). */
+ * PCF_RNG_HOST: host_normals[n*M+m] are the draws N(0, sqrt(T/M)); with
+ * kernel_variant 0 the device replays the reference's FP64 arithmetic on them,
+ * with kernel_variant 1 it runs the production FP64 step (I += S (a1 + a2 z),
+ * S *= exp(mu + vol z)) on z = draw / sqrt(T/M), the arithmetic of the Philox
+ * forms on the reference's stream. */
 int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
                 int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
                 pcf_result* res);
@@ -219,6 +232,14 @@ int pcf_autotune(pcf_ctx* ctx, int32_t kind_mask, int32_t max_log2_paths, pcf_tu
 /* Forgets every tuned shape (back to the built-in defaults). */
 void pcf_tune_reset(void);
 
+/* Testing hook for the process-wide basket factor cache (no counterpart in
+ * the reference): the cache finds an entry by a 64-bit hash of the basket's
+ * parameters ANDed with `mask` (default all ones) and uses it only when the
+ * entry's full parameter key equals the request's.  mask = 0 sends every
+ * basket to one hash value, so tests can show that collisions never mix
+ * factors. */
+void pcf_debug_basket_hash_mask(uint64_t mask);
+
 /* Standard normals of the device generator: out[p*K + k] = normal k of path
  * path0+p of `stream` (the exact values the pricing kernels consume, in FP64
  * or rounded to FP32 per `precision`).  For reproducibility checks. */
@@ -268,6 +289,29 @@ int pcf_server_wait(pcf_server* srv, uint64_t ticket, pcf_result* res,
 int pcf_server_destroy(pcf_server* srv);
 /* Message of the last refused submit or failed launch. */
 const char* pcf_server_last_error(pcf_server* srv);
+
+/* Device memory of a server.  Each slot is charged for the request-sized
+ * buffers it holds or may hold (path stores and per-path state, reduction
+ * partials, binomial tables): what it kept from its last request when idle,
+ * plus the footprint of the unit it runs when busy.  The sum of the charges
+ * never exceeds mem_budget_bytes: kept buffers are freed (the slot's own,
+ * then idle slots' least recently used) before a request waits. */
+typedef struct pcf_server_stats {
+    int64_t budget_bytes;         /* mem_budget_bytes in force */
+    int64_t charged_bytes;        /* sum of the slots' charges now */
+    int64_t peak_charged_bytes;   /* largest charged_bytes so far (<= budget_bytes) */
+    int64_t held_bytes;           /* request-sized bytes the slots held when they last went idle */
+    int64_t peak_held_bytes;      /* largest held_bytes at a unit's completion */
+    int64_t fixed_bytes_per_slot; /* per-slot buffers outside the budget (arrival counters,
+                                     results, job table, basket factor slots) */
+    int64_t evictions;            /* kept buffers freed to admit a request (a slot's own, or
+                                     an idle slot's) */
+    int64_t underestimates;       /* units that held more than their charge (then freed); 0 */
+    int64_t completed;            /* requests completed */
+    int32_t slots;
+    int32_t reserved[5];
+} pcf_server_stats;
+int pcf_server_get_stats(pcf_server* srv, pcf_server_stats* out);
 
 #ifdef __cplusplus
 }

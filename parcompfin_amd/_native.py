@@ -60,7 +60,8 @@ class PcfOpts(C.Structure):
         ("when_out", C.POINTER(C.c_int32)),
         ("kernel_variant", C.c_int32),
         ("lsm_basis", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("fp64_check", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -83,7 +84,8 @@ class PcfResult(C.Structure):
         ("grid_blocks", C.c_int32),
         ("block_size", C.c_int32),
         ("batch_size", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("fp64_checked", C.c_int32),
+        ("fp64_err", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -149,6 +151,25 @@ class PcfRequestTiming(C.Structure):
     ]
 
 
+class PcfServerStats(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_int64),
+        ("charged_bytes", C.c_int64),
+        ("peak_charged_bytes", C.c_int64),
+        ("held_bytes", C.c_int64),
+        ("peak_held_bytes", C.c_int64),
+        ("fixed_bytes_per_slot", C.c_int64),
+        ("evictions", C.c_int64),
+        ("underestimates", C.c_int64),
+        ("completed", C.c_int64),
+        ("slots", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
 _d = C.c_double
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -178,8 +199,10 @@ SIGNATURES = {
     "pcf_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(PcfResult), C.POINTER(PcfRequestTiming)]),
     "pcf_server_destroy": (C.c_int, [C.c_void_p]),
     "pcf_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "pcf_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(PcfServerStats)]),
     "pcf_autotune": (C.c_int, [_ctx, _i32, _i32, C.POINTER(PcfTuneEntry), _i32, C.POINTER(_i32)]),
     "pcf_tune_reset": (None, []),
+    "pcf_debug_basket_hash_mask": (None, [C.c_uint64]),
 }
 
 STRUCTS = {
@@ -190,6 +213,7 @@ STRUCTS = {
     "pcf_server_config": PcfServerConfig,
     "pcf_request_timing": PcfRequestTiming,
     "pcf_tune_entry": PcfTuneEntry,
+    "pcf_server_stats": PcfServerStats,
 }
 
 _lib = None

@@ -58,6 +58,8 @@ class Result:
     grid_blocks: int
     block_size: int
     batch_size: int
+    fp64_checked: int
+    fp64_err: float
     extra: dict = field(default_factory=dict)
 
     @classmethod
@@ -114,7 +116,8 @@ class Engine:
 
     @staticmethod
     def make_opts(seed=0x5EED, precision="fp32k", normals=None, ref_exact=False, control=False,
-                  block_size=0, items_per_thread=0, when_out=None, kernel_variant=0, lsm_basis=0):
+                  block_size=0, items_per_thread=0, when_out=None, kernel_variant=0, lsm_basis=0,
+                  fp64_check=False):
         o = N_.PcfOpts()
         N_.load().pcf_opts_init(C.byref(o))
         o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
@@ -132,6 +135,7 @@ class Engine:
         o.items_per_thread = int(items_per_thread)
         o.kernel_variant = int(kernel_variant)
         o.lsm_basis = int(lsm_basis)
+        o.fp64_check = 1 if fp64_check else 0
         if when_out is not None:
             keep.append(when_out)
             o.when_out = when_out.ctypes.data_as(C.POINTER(C.c_int32))
@@ -290,6 +294,14 @@ class Server:
             raise N_.PcfError(st, self.last_error())
         timing = {f: getattr(tm, f) for f, _ in N_.PcfRequestTiming._fields_}
         return Result.from_c(res, timing=timing)
+
+    def stats(self) -> dict:
+        """Device-memory accounting of the server (pcf_server_get_stats)."""
+        s = N_.PcfServerStats()
+        st = self.lib.pcf_server_get_stats(self.srv, C.byref(s))
+        if st != N_.PCF_OK:
+            raise N_.PcfError(st, self.last_error())
+        return s.as_dict()
 
     def close(self):
         if getattr(self, "srv", None):

@@ -19,13 +19,13 @@ struct AsiaArgs {
     double mu, vol;         // (r - s^2/2) dt, sigma sqrt(dt)
     double h_c1, h_mu, sigma;  // verification: 1 + r*dt/2, (r - pow(s,2)/2)*dt, sigma
     float S0f, Ef, pff, a1f, a2f, mu2f, vol2f;  // FP32; exponent constants x log2(e)
-    double invM;
+    double invM, sdt;       // 1/M, sqrt(dt)
     int64_t N;
     int32_t M;
     uint64_t seed;
 };
 
-enum { kFp32k = 0, kFp64 = 1, kHost = 2 };
+enum { kFp32k = 0, kFp64 = 1, kHost = 2, kHostProd = 3 };
 
 template <bool CONTROL>
 __device__ __forceinline__ void asia_path_f32(const AsiaArgs& a, uint64_t p, float& pay,
@@ -100,6 +100,17 @@ __device__ __forceinline__ void asia_path_f64(const AsiaArgs& a, uint64_t p, dou
     if (CONTROL) geo = pcf::payoff<double>(a.S0 * exp(lsum / (double)a.M), a.E, a.pf);
 }
 
+// The production FP64 step of asia_path_f64 on the reference's draws z = dB/sqrt(dt).
+__device__ __forceinline__ double asia_path_prod_host(const AsiaArgs& a, const double* __restrict__ w) {
+    double S = a.S0, I = 0.0;
+    for (int m = 0; m < a.M; ++m) {
+        const double z = w[m] / a.sdt;
+        I += S * fma(a.a2, z, a.a1);
+        S *= exp(fma(a.vol, z, a.mu));
+    }
+    return pcf::payoff<double>(I / (double)a.M, a.E, a.pf);
+}
+
 // Verification: the reference's expression order, FP64, no contraction.
 __device__ __forceinline__ double asia_path_ref(const AsiaArgs& a, const double* __restrict__ z) {
 #pragma clang fp contract(off)
@@ -140,6 +151,8 @@ __device__ __forceinline__ void asia_body(const AsiaArgs& a, const double* __res
             gv = pg;
         } else if (MODE == kFp64) {
             asia_path_f64<CONTROL>(a, (uint64_t)p, v, gv);
+        } else if (MODE == kHostProd) {
+            v = asia_path_prod_host(a, hz + p * (int64_t)a.M);
         } else {
             v = asia_path_ref(a, hz + p * (int64_t)a.M);
         }
@@ -207,11 +220,16 @@ int asia_prepare(pcf_ctx* ctx, const pcf_request& q, AsiaPrep& p) {
     a.mu2f = (float)(a.mu * log2e);
     a.vol2f = (float)(a.vol * log2e);
     a.invM = 1.0 / (double)M;
+    a.sdt = sdt;
     a.N = q.N;
     a.M = M;
     a.seed = o.seed;
-    p.mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
-    p.control = o.control != 0 && p.mode != kHost;
+    if (o.rng == PCF_RNG_HOST && o.kernel_variant != 0 && o.kernel_variant != 1)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "mc_asia with host normals: kernel_variant 0 (reference "
+                        "arithmetic) or 1 (production arithmetic), got %d", o.kernel_variant);
+    p.mode = o.rng == PCF_RNG_HOST ? (o.kernel_variant == 1 ? kHostProd : kHost)
+                                   : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
+    p.control = o.control != 0 && p.mode != kHost && p.mode != kHostProd;
     int B = 256, ipt = 1;
     pcf_tuned_shape(PCF_KIND_ASIA, p.mode, q.N, &B, &ipt);
     if (o.block_size) B = o.block_size;
@@ -220,6 +238,32 @@ int asia_prepare(pcf_ctx* ctx, const pcf_request& q, AsiaPrep& p) {
     p.nblk = pcf_blocks_for(q.N, B, ipt);
     p.disc = std::exp(-q.r * q.T);
     return PCF_OK;
+}
+
+// Result record of one request from its 8 output doubles (write_moments:
+// sum, sum of squares, mean, variance of the payoffs, then of the control).
+void asia_result(const AsiaPrep& p, const double* h, int64_t N, int M, int B, int batch, const pcf_result& timing,
+                 pcf_result& r) {
+    // The reference returns exp(-r*T)*result/(double)N (src/mc_asia.cpp:39).
+    r.price = p.disc * h[0] / (double)N;
+    r.mean = h[2];
+    r.variance = h[3];
+    r.std_err = p.disc * std::sqrt(h[3] / (double)N);
+    if (p.control) {
+        r.control_price = p.disc * h[4] / (double)N;
+        r.control_se = p.disc * std::sqrt(h[7] / (double)N);
+    }
+    r.n_paths = N;
+    r.n_steps = M;
+    r.kernel_ms = timing.kernel_ms;
+    r.total_ms = timing.total_ms;
+    // 14 algorithmic flops per path-step: normal 6, I update 2 FMAs (4),
+    // S update FMA + exp + mul (4).  No HBM traffic in the Philox forms.
+    r.flops = 14.0 * (double)N * (double)M;
+    r.bytes = p.mode >= kHost ? 8.0 * (double)N * (double)M : 0.0;
+    r.grid_blocks = p.nblk;
+    r.block_size = B;
+    r.batch_size = batch;
 }
 
 template <int MODE, bool CONTROL>
@@ -257,7 +301,7 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "mixed batch keys");
         nparts += (int64_t)p.nblk * K;
         nb += p.nblk;
-        if (p.mode == kHost) nnorm += reqs[live[k]]->N * (int64_t)reqs[live[k]]->M;
+        if (p.mode >= kHost) nnorm += reqs[live[k]]->N * (int64_t)reqs[live[k]]->M;
     }
     if (nb > (1ll << 30)) return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "batch too large");
     int st = pcf_timing_begin(ctx);
@@ -273,7 +317,7 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
         AsiaJob& J = jobs[k];
         J.a = p.a;
         J.hz = nullptr;
-        if (p.mode == kHost) {
+        if (p.mode >= kHost) {
             const pcf_opts& o = reqs[i]->opts;
             const int64_t cnt = reqs[i]->N * (int64_t)reqs[i]->M;
             if (!o.host_normals || o.n_host_normals < cnt)
@@ -301,8 +345,10 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
     } else if (mode == kFp64) {
         if (control) launch_asia<kFp64, true>(ctx, boff, B, nl);
         else launch_asia<kFp64, false>(ctx, boff, B, nl);
-    } else {
+    } else if (mode == kHost) {
         launch_asia<kHost, false>(ctx, boff, B, nl);
+    } else {
+        launch_asia<kHostProd, false>(ctx, boff, B, nl);
     }
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
@@ -310,31 +356,7 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
     if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
     for (int k = 0; k < nl; ++k) {
         const int i = live[k];
-        const AsiaPrep& p = prep[i];
-        const double* h = ctx->h_out + (size_t)kOutPerJob * k;
-        const int64_t N = reqs[i]->N;
-        const int M = reqs[i]->M;
-        pcf_result& r = res[i];
-        // The reference returns exp(-r*T)*result/(double)N (src/mc_asia.cpp:39).
-        r.price = p.disc * h[0] / (double)N;
-        r.mean = h[2];
-        r.variance = h[3];
-        r.std_err = p.disc * std::sqrt(h[3] / (double)N);
-        if (p.control) {
-            r.control_price = p.disc * h[4] / (double)N;
-            r.control_se = p.disc * std::sqrt(h[7] / (double)N);
-        }
-        r.n_paths = N;
-        r.n_steps = M;
-        r.kernel_ms = timing.kernel_ms;
-        r.total_ms = timing.total_ms;
-        // 14 algorithmic flops per path-step: normal 6, I update 2 FMAs (4),
-        // S update FMA + exp + mul (4).  No HBM traffic in the Philox forms.
-        r.flops = 14.0 * (double)N * (double)M;
-        r.bytes = p.mode == kHost ? 8.0 * (double)N * (double)M : 0.0;
-        r.grid_blocks = p.nblk;
-        r.block_size = B;
-        r.batch_size = nl;
+        asia_result(prep[i], ctx->h_out + (size_t)kOutPerJob * k, reqs[i]->N, reqs[i]->M, B, nl, timing, res[i]);
     }
     return PCF_OK;
 }
@@ -349,5 +371,6 @@ extern "C" int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double s
     const pcf_request* qp = &q;
     int status = PCF_OK;
     int st = pcf_asia_run(ctx, 1, &qp, res, &status);
-    return st ? st : status;
+    if (st || status) return st ? st : status;
+    return pcf_apply_fp64_check(ctx, q, res);
 }

@@ -16,6 +16,7 @@
 //    never touch memory; the epilogue (exp, weights, basket sum) runs on the
 //    16 accumulator registers, halves combined with one lane exchange.
 #include <cmath>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -304,12 +305,26 @@ bool eigen_factor(int d, const std::vector<double>& C, std::vector<double>& F) {
 // ---- host: factor cache, preparation, batched launch ------------------------
 
 // Process-wide cache of factorised baskets (Cholesky or eigen factor plus the
-// per-asset constants), keyed by a hash of everything they depend on.
+// per-asset constants), found by a hash of everything they depend on; an
+// entry keeps that full key and is used only when it equals the request's
+// (a hash collision is a miss).  id is unique per entry and never reused, so
+// the contexts' device slots are keyed by it.
 struct BasketConsts {
     std::vector<char> blob;
+    std::vector<double> key;
+    uint64_t id = 0;
     BasketLayout lay;
     bool full;
 };
+
+std::atomic<uint64_t>& consts_next_id() {
+    static std::atomic<uint64_t> n{1};
+    return n;
+}
+std::atomic<uint64_t>& consts_hash_mask() {
+    static std::atomic<uint64_t> m{~0ull};
+    return m;
+}
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
     const unsigned char* c = (const unsigned char*)p;
@@ -333,7 +348,7 @@ struct BasketPrep {
     BasketArgs a;
     int mode, B, nblk, dmax;
     bool mfma, full;
-    uint64_t key;
+    uint64_t key;  // the constants' entry id (device slots are keyed by it)
     std::shared_ptr<BasketConsts> consts;
     double disc;
 };
@@ -348,28 +363,25 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
         return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "assets must be in [1, %d] (got %d)", kMaxAssets, d);
     const pcf_basket_ext* ext = q.basket;
     // Key of the constants: d, rho, r, T, sqrt(T) convention, scalar sigma/S0
-    // and every extension array.
-    uint64_t h = 1469598103934665603ull;
+    // and every extension array (absent arrays marked by a NaN-free flag).
     const int refx = o.ref_exact ? 1 : 0;
-    h = fnv(h, &d, sizeof d);
-    h = fnv(h, &q.rho, sizeof q.rho);
-    h = fnv(h, &q.r, sizeof q.r);
-    h = fnv(h, &q.T, sizeof q.T);
-    h = fnv(h, &q.sigma, sizeof q.sigma);
-    h = fnv(h, &q.S0, sizeof q.S0);
-    h = fnv(h, &refx, sizeof refx);
     const double* arrs[4] = {ext ? ext->sigma : nullptr, ext ? ext->S0 : nullptr,
                              ext ? ext->weights : nullptr, ext ? ext->corr : nullptr};
+    std::vector<double> key = {(double)d, q.rho, q.r, q.T, q.sigma, q.S0, (double)refx};
     for (int k = 0; k < 4; ++k) {
-        const int present = arrs[k] ? 1 : 0;
-        h = fnv(h, &present, sizeof present);
-        if (arrs[k]) h = fnv(h, arrs[k], sizeof(double) * (k == 3 ? (size_t)d * d : (size_t)d));
+        key.push_back(arrs[k] ? 1.0 : 0.0);
+        if (arrs[k]) key.insert(key.end(), arrs[k], arrs[k] + (k == 3 ? (size_t)d * d : (size_t)d));
     }
+    const uint64_t h = fnv(1469598103934665603ull, key.data(), key.size() * sizeof(double)) &
+                       consts_hash_mask().load(std::memory_order_relaxed);
     std::shared_ptr<BasketConsts> c;
     {
         std::lock_guard<std::mutex> lk(consts_mu());
         auto it = consts_cache().find(h);
-        if (it != consts_cache().end()) c = it->second;
+        // bitwise comparison of the full key: a hash collision is a miss
+        if (it != consts_cache().end() && it->second->key.size() == key.size() &&
+            std::memcmp(it->second->key.data(), key.data(), key.size() * sizeof(double)) == 0)
+            c = it->second;
     }
     if (!c) {
         // Correlation matrix: ext->corr or equicorrelation rho (include/mvn.h:55-60).
@@ -393,6 +405,8 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
         }
         c = std::make_shared<BasketConsts>();
         c->full = full;
+        c->key = key;
+        c->id = consts_next_id().fetch_add(1);
         c->lay = BasketLayout::make(d);
         c->blob.assign(c->lay.bytes, 0);
         char* hb = c->blob.data();
@@ -435,7 +449,7 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
         consts_cache()[h] = c;
     }
     p.consts = c;
-    p.key = h;
+    p.key = c->id;
     p.full = c->full;
     p.mode = o.rng == PCF_RNG_HOST ? kHost : (o.precision == PCF_FP64 ? kFp64 : kFp32k);
     if (o.kernel_variant == 2 && !(p.mode == kFp32k && d >= 16 && d <= 32))
@@ -470,7 +484,7 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
 }
 
 // Device-resident constants: fixed-size slots in ctx->basket_L, LRU by stamp.
-constexpr size_t kConstSlot = 64 * 1024;
+constexpr size_t kConstSlot = kBasketConstSlot;
 constexpr int kConstSlots = kMaxBatch;
 
 int resident_consts(pcf_ctx* ctx, const BasketPrep& p, uint64_t batch_stamp, char** dev) {
@@ -646,5 +660,10 @@ extern "C" int pcf_mc_eur_multi(pcf_ctx* ctx, double S0, double E, double r, dou
     const pcf_request* qp = &q;
     int status = PCF_OK;
     int st = pcf_basket_run(ctx, 1, &qp, res, &status);
-    return st ? st : status;
+    if (st || status) return st ? st : status;
+    return pcf_apply_fp64_check(ctx, q, res);
+}
+
+extern "C" void pcf_debug_basket_hash_mask(uint64_t mask) {
+    consts_hash_mask().store(mask, std::memory_order_relaxed);
 }

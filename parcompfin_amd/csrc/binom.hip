@@ -19,10 +19,17 @@
 //    wavefront: the block stages the table through LDS and every wave reads
 //    its values as LDS broadcasts; lanes that have reached their own end are
 //    masked, never reordered.
-//  * lgamma (production): lgamma(N+1) - lgamma(i+1) - lgamma(N-i+1) in FP64.
+//  * double-double (production): one lane per term i = 0..N, weight
+//        t = lf[N] - lf[i] - lf[N-i] + i log p + (N-i) log q,   w = exp(t)
+//    in double-double arithmetic from a resident table lf[k] = log k! (hi,
+//    lo).  The FP64 forms of these logs round at ulp(log N!) = 1.9e-9 for
+//    N = 1e6; the reference's own chain carries that rounding into a price
+//    1.5e-8 from the exact sum, while this form stays within 1e-15 of it.
 #include <cmath>
+#include <mutex>
 #include <vector>
 
+#include "pcf_dd.hpp"
 #include "pcf_internal.hpp"
 
 namespace {
@@ -133,23 +140,84 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     }
 }
 
-__global__ __launch_bounds__(256) void k_binom_lgamma(BinomArgs a, pcf::CSum* __restrict__ partials,
-                                                      unsigned int* __restrict__ counter,
-                                                      double* __restrict__ out) {
+struct BinomDDArgs {
+    double S0, E, pf, u, d;
+    pcf::DD lp, lq;  // log p, log q
+    int64_t N;
+};
+
+// Double-double form: lane i prices term i (0 <= i <= N).  lf holds log k! as
+// (hi, lo) pairs; the weight's logs are combined without rounding error
+// beyond ~1e-32 relative, then w = e^{t.hi} (1 + t.lo).
+__global__ __launch_bounds__(256) void k_binom_dd(BinomDDArgs a, const double2* __restrict__ lf,
+                                                  pcf::CSum* __restrict__ partials,
+                                                  unsigned int* __restrict__ counter,
+                                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
     __shared__ pcf::CSum lds[4];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0, s_b = 0.0;
-    if (i < a.lanes) {
-        const double lgN = lgamma((double)a.N + 1.0);
-        const double comb_val = lgN - lgamma((double)i + 1.0) - lgamma((double)(a.N - i) + 1.0);
-        s = term_value(a, comb_val, i);
-        if (i < a.until) s_b = term_value(a, comb_val, a.N - i);
+    double s = 0.0;
+    if (i <= a.N) {
+        const double2 fN = lf[a.N], fi = lf[i], fk = lf[a.N - i];
+        pcf::DD t = pcf::dd_add(pcf::DD{fN.x, fN.y}, pcf::DD{-fi.x, -fi.y});
+        t = pcf::dd_add(t, pcf::DD{-fk.x, -fk.y});
+        t = pcf::dd_add(t, pcf::dd_mul_d(a.lp, (double)i));
+        t = pcf::dd_add(t, pcf::dd_mul_d(a.lq, (double)(a.N - i)));
+        const double e = exp(t.h);
+        const double w = e + e * t.l;
+        if (w != 0.0) {
+            const double S = a.S0 * pow(a.u, (double)i) * pow(a.d, (double)(a.N - i));
+            const double v = a.pf * (S - a.E);
+            s = w * (v > 0.0 ? v : 0.0);
+        }
     }
     pcf::Acc<1> ac, tot;
     ac.v[0] = {s, 0.0};
-    pcf::csum_add(ac.v[0], s_b);
     if (pcf::grid_reduce<1>(ac, partials, counter, lds, tot, blockIdx.x, gridDim.x) && threadIdx.x == 0)
         out[0] = tot.v[0].s + tot.v[0].c;
+}
+
+// Process-wide host table lf[k] = log k! (hi, lo), k = 0..n, grown on demand
+// and never shrunk.  Built from IEEE double operations only:
+//   log 2 as a double-double constant; log k = log(k-1) + 2 atanh(1/(2k-1))
+//   for k >= 3 (series in double-double); lf[k] = lf[k-1] + log k.
+// Each step rounds at ~1e-32 relative, so lf[10^6] ~ 1.3e7 is within ~1e-20
+// of log 10^6!.
+struct LogFactTable {
+    std::mutex mu;
+    std::vector<double> v;           // 2 (n + 1) doubles
+    pcf::DD Lk{0.0, 0.0}, lf{0.0, 0.0};  // log k and log k! of the last entry
+    int64_t n = -1;
+};
+LogFactTable& logfact_table() {
+    static LogFactTable t;
+    return t;
+}
+
+// Makes ctx->logfact hold lf[0..N] (at least).  Resident: a context keeps the
+// largest table it was asked for.
+int ensure_logfact(pcf_ctx* ctx, int64_t N) {
+    if (ctx->logfact_n >= N) return PCF_OK;
+    LogFactTable& T = logfact_table();
+    std::lock_guard<std::mutex> lk(T.mu);
+    if (T.n < N) {
+        T.v.resize(2 * ((size_t)N + 1));
+        for (int64_t k = T.n + 1; k <= N; ++k) {
+            if (k >= 2) {
+                if (k == 2) T.Lk = pcf::kDdLn2;
+                else T.Lk = pcf::dd_add(T.Lk, pcf::dd_2atanh(pcf::dd_div_d(pcf::DD{1.0, 0.0}, (double)(2 * k - 1))));
+                T.lf = pcf::dd_add(T.lf, T.Lk);
+            }
+            T.v[2 * (size_t)k] = T.lf.h;
+            T.v[2 * (size_t)k + 1] = T.lf.l;
+        }
+        T.n = N;
+    }
+    const size_t bytes = 2 * ((size_t)T.n + 1) * sizeof(double);
+    PCF_HIP_CHECK(ctx, ctx->logfact.ensure(bytes));
+    PCF_HIP_CHECK(ctx, hipMemcpy(ctx->logfact.ptr, T.v.data(), bytes, hipMemcpyHostToDevice));
+    ctx->logfact_n = T.n;
+    return PCF_OK;
 }
 
 // glibc log(n) for n = 0..N (entry 0 unused) and the descending chain's
@@ -214,10 +282,20 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     a.lanes = a.until + ((N % 2 == 0) ? 1 : 0);
 
     const int B = 256;
-    const int grid = (int)((a.lanes + B - 1) / B);
     const bool chain = o.ref_exact != 0;
+    const int grid = (int)(((chain ? a.lanes : N + 1) + B - 1) / B);
+    BinomDDArgs ad{};
+    ad.S0 = S0;
+    ad.E = E;
+    ad.pf = payoff_fun;
+    ad.u = u;
+    ad.d = d;
+    ad.lp = pcf::dd_log(p);
+    ad.lq = pcf::dd_log(q);
+    ad.N = N;
     if ((st = pcf_timing_begin(ctx))) return st;
     if (chain && (st = ensure_logtab(ctx, N))) return st;
+    if (!chain && (st = ensure_logfact(ctx, N))) return st;
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
@@ -226,27 +304,27 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
                                                   ctx->counters.as<unsigned int>(),
                                                   ctx->out.as<double>());
     else
-        k_binom_lgamma<<<grid, B, 0, ctx->stream>>>(a, ctx->partials.as<pcf::CSum>(),
-                                                   ctx->counters.as<unsigned int>(),
-                                                   ctx->out.as<double>());
+        k_binom_dd<<<grid, B, 0, ctx->stream>>>(ad, ctx->logfact.as<double2>(), ctx->partials.as<pcf::CSum>(),
+                                               ctx->counters.as<unsigned int>(), ctx->out.as<double>());
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
     if ((st = pcf_timing_end(ctx, 1, res))) return st;
     // The reference returns exp(-r*T)*result (src/binom_embar.cpp:49).
     res->mean = ctx->h_out[0];
     res->price = std::exp(-r * T) * ctx->h_out[0];
-    res->n_paths = a.lanes;
+    res->n_paths = chain ? a.lanes : N + 1;
     res->n_steps = N;
     // chain: the reference's 0.75 N^2 log-adds become table adds
-    // (sum over lanes of (N-i) + (N-i-1)); lgamma: ~20 flops per term.
+    // (sum over lanes of (N-i) + (N-i-1)); double-double: ~20 algorithmic
+    // flops per term (the reference's term arithmetic, counted once).
     double adds = 0.0;
     if (chain) {
         const double L = (double)a.until;
         adds = 2.0 * (double)N * L - L * (L - 1.0) - L;  // sum_{i<until} (2N - 2i - 1)
         if (N % 2 == 0) adds += (double)N - 1.0;         // middle lane
     }
-    res->flops = chain ? adds + 24.0 * (double)a.lanes : 40.0 * (double)a.lanes;
-    res->bytes = chain ? 8.0 * adds : 0.0;
+    res->flops = chain ? adds + 24.0 * (double)a.lanes : 20.0 * (double)(N + 1);
+    res->bytes = chain ? 8.0 * adds : 48.0 * (double)(N + 1);  // three 16-byte table entries per term
     res->grid_blocks = grid;
     res->block_size = B;
     res->batch_size = 1;

@@ -70,8 +70,13 @@ int pcf_ctx_create(int device, pcf_ctx** out) {
         return bail(e, "alloc counters");
     if ((e = ctx->out.ensure(kOutDoubles * sizeof(double), true)) != hipSuccess)
         return bail(e, "alloc out");
-    if ((e = hipHostMalloc((void**)&ctx->h_out, kOutDoubles * sizeof(double), 0)) != hipSuccess)
+    // Coherent: the persistent LSM kernel writes its results here directly,
+    // and the host reads them once the launch has completed.
+    if ((e = hipHostMalloc((void**)&ctx->h_out, kOutDoubles * sizeof(double), hipHostMallocCoherent)) !=
+        hipSuccess)
         return bail(e, "hipHostMalloc");
+    if ((e = hipHostGetDevicePointer((void**)&ctx->h_out_dev, ctx->h_out, 0)) != hipSuccess)
+        return bail(e, "hipHostGetDevicePointer");
     if ((e = hipHostMalloc(&ctx->h_jobs, kMaxBatch * kJobBytes, 0)) != hipSuccess)
         return bail(e, "hipHostMalloc jobs");
     if ((e = ctx->jobs.ensure(kMaxBatch * kJobBytes)) != hipSuccess) return bail(e, "alloc jobs");
@@ -85,7 +90,7 @@ int pcf_ctx_destroy(pcf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     pcf_lsm_release(ctx);
     for (DevBuf* b : {&ctx->partials, &ctx->counters, &ctx->out, &ctx->normals, &ctx->logtab,
-                      &ctx->prefix, &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef,
+                      &ctx->prefix, &ctx->logfact, &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef,
                       &ctx->basket_L, &ctx->scratch, &ctx->jobs, &ctx->lattice})
         b->release();
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
@@ -157,6 +162,38 @@ int pcf_upload_normals(pcf_ctx* ctx, const pcf_opts* o, int64_t count) {
     PCF_HIP_CHECK(ctx, ctx->normals.ensure(bytes));
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->normals.ptr, o->host_normals, bytes,
                                       hipMemcpyHostToDevice, ctx->stream));
+    return PCF_OK;
+}
+
+int pcf_apply_fp64_check(pcf_ctx* ctx, const pcf_request& q, pcf_result* res) {
+    const pcf_opts& o = q.opts;
+    if (!o.fp64_check || o.precision != PCF_FP32K || o.rng != PCF_RNG_PHILOX) return PCF_OK;
+    pcf_opts o64 = o;
+    o64.precision = PCF_FP64;
+    o64.fp64_check = 0;
+    o64.when_out = nullptr;
+    pcf_result r64{};
+    int st;
+    switch (q.kind) {
+        case PCF_KIND_EUR:
+            st = pcf_mc_eur(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun, &o64, &r64);
+            break;
+        case PCF_KIND_ASIA:
+            st = pcf_mc_asia(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.M, q.payoff_fun, &o64, &r64);
+            break;
+        case PCF_KIND_AMER:
+            st = pcf_mc_amer(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.M, q.payoff_fun, &o64, &r64);
+            break;
+        case PCF_KIND_BASKET:
+            st = pcf_mc_eur_multi(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun, q.assets, q.rho, q.basket,
+                                  &o64, &r64);
+            break;
+        default:
+            return PCF_OK;
+    }
+    if (st) return st;
+    res->fp64_err = std::fabs(res->price - r64.price);
+    res->fp64_checked = 1;
     return PCF_OK;
 }
 
@@ -237,5 +274,6 @@ extern "C" int pcf_struct_size(const char* name) {
     if (n == "pcf_server_config") return (int)sizeof(pcf_server_config);
     if (n == "pcf_request_timing") return (int)sizeof(pcf_request_timing);
     if (n == "pcf_tune_entry") return (int)sizeof(pcf_tune_entry);
+    if (n == "pcf_server_stats") return (int)sizeof(pcf_server_stats);
     return -1;
 }

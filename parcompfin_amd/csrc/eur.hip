@@ -246,5 +246,6 @@ extern "C" int pcf_mc_eur(pcf_ctx* ctx, double S0, double E, double r, double si
     const pcf_request* qp = &q;
     int status = PCF_OK;
     int st = pcf_eur_run(ctx, 1, &qp, res, &status);
-    return st ? st : status;
+    if (st || status) return st ? st : status;
+    return pcf_apply_fp64_check(ctx, q, res);
 }

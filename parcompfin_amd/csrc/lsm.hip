@@ -1146,12 +1146,36 @@ __device__ __forceinline__ void lsm_stage_row(Real* rowbuf, const Real* __restri
 // needs M + 1 <= kDiscLds; longer schedules run as the graph of launches).
 constexpr int kDiscLds = 1024;
 
+// Everything a persistent launch reads besides its buffers travels in its
+// kernel arguments: the parameter block and, for M + 1 <= kDiscArgs, the
+// discount table.  A request then needs no copy before its launch.
+constexpr int kDiscArgs = 400;
+struct LsmPersistArgs {
+    LsmBufs b;
+    LsmSync sy;
+    LsmParams P;
+    double disc[kDiscArgs];  // exp(-r dt k), k = 0..M; longer tables are read from b.disc
+};
+static_assert(sizeof(LsmPersistArgs) <= 4096, "kernel arguments are limited to 4 KB");
+
+// Results of a persistent launch, written by the last block to arrive at the
+// final sums: out[0..3] = European control sum and sum of squares, discounted
+// cash-flow sum and sum of squares; out[4..6] = dates with 1-2 in-the-money
+// paths, with a reduced basis, with a singular reference Gram matrix;
+// out[7] = 1 if a date barrier wait gave up.  The same block then clears the
+// sync words, so the next launch needs no fill.
+constexpr int kPersistOut = 8;
+static_assert(kPersistOut <= kOutDoubles, "the persistent results fit the context's pinned result buffer");
+
 // K: regression basis size (3: the quadratic of lsm_solve_g, else the K-term
 // basis of lsm_solve_poly).
 template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
-__global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) {
+__global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs args) {
     using Bs = LsmBasis<K>;
     static_assert(K == 3 || !REF, "the reference-exact mode keeps the reference's basis");
+    const LsmBufs& b = args.b;
+    const LsmSync& sy = args.sy;
+    int n_small = 0, n_reduced = 0, n_singular = 0;  // date flags (the same in every thread)
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
     __shared__ double red_a[16 * 8 * Bs::kChunks], red_b[16 * 8 * Bs::kChunks], tot_sh[8 * Bs::kChunks];
@@ -1159,13 +1183,16 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     extern __shared__ __align__(16) unsigned char lsm_dyn_lds[];  // row staging (kLdsRow)
     // A private copy: the row copies into LDS count as memory writes, and
     // through a reference every use would reload (and wait for) the block.
-    const LsmParams P = *b.P;
+    const LsmParams P = args.P;
     Real* __restrict__ store = (Real*)b.store;
     const int64_t N = P.N, half = P.half;
     const int M = P.M;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int k = threadIdx.x; k <= M; k += blockDim.x) disc[k] = b.disc[k];
+    if (M + 1 <= kDiscArgs)
+        for (int k = threadIdx.x; k <= M; k += blockDim.x) disc[k] = args.disc[k];
+    else
+        for (int k = threadIdx.x; k <= M; k += blockDim.x) disc[k] = b.disc[k];
     // Date slots: 2 parities x grid x 8 C doubles, then the final sums' slots.
     double* __restrict__ slots = (double*)b.partials;
     __syncthreads();
@@ -1357,6 +1384,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
+            n_small += (int)cm[0] == kFlagSmall;
+            n_reduced += (int)cm[0] == kFlagReduced;
+            n_singular += (int)cm[0] == kFlagSingular;
             if ((int)cm[0] == kFlagSmall || (int)cm[0] == kFlagRegress || (int)cm[0] == kFlagReduced) {
 #pragma unroll
                 for (int k = 0; k < PMAX; ++k) {
@@ -1383,6 +1413,8 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k <= K; ++k) b.coef[Bs::kCoef * m + k] = cm[k];
+            n_small += (int)cm[0] == kFlagSmall;
+            n_reduced += (int)cm[0] == kFlagReduced;
 #pragma unroll
             for (int k = 0; k < PMAX; ++k) {
                 const int64_t n = t0 + (int64_t)k * T;
@@ -1427,11 +1459,21 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(LsmBufs b, LsmSync sy) 
     pcf::Acc<4> a, tot;
 #pragma unroll
     for (int k = 0; k < 4; ++k) a.v[k] = {threadIdx.x == 0 ? fin[k] : 0.0, 0.0};
-    // Final sums: last-arriving block, on its own counter and slots.
-    if (lsm_arrive<4>(a, (pcf::CSum*)(slots + (size_t)2 * gridDim.x * (8 * Bs::kChunks)), sy.fin, lds, tot) &&
-        threadIdx.x == 0) {
+    // Final sums: last-arriving block, on its own counter and slots.  Every
+    // block has passed its last date barrier before it arrives here, so the
+    // last one may read the error word and return all sync words to zero.
+    if (lsm_arrive<4>(a, (pcf::CSum*)(slots + (size_t)2 * gridDim.x * (8 * Bs::kChunks)), sy.fin, lds, tot)) {
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) b.out[k] = tot.v[k].s + tot.v[k].c;
+            for (int k = 0; k < 4; ++k) b.out[k] = tot.v[k].s + tot.v[k].c;
+            b.out[4] = (double)n_small;
+            b.out[5] = (double)n_reduced;
+            b.out[6] = (double)n_singular;
+            b.out[7] = pcf::fetch_u32(sy.err) != 0u ? 1.0 : 0.0;
+            pcf::publish_u32(sy.err, 0u);
+            pcf::publish_u32(sy.fin, 0u);
+        }
+        if (threadIdx.x < kArrivalCounters) pcf::publish_u32(sy.arrive + threadIdx.x * kCounterStride, 0u);
     }
 }
 
@@ -1443,10 +1485,9 @@ size_t lsm_dyn_lds_bytes(int threads) {
 }
 
 template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
-hipError_t launch_persistent_t(const LsmBufs& b, const LsmSync& sy, int grid, int B, hipStream_t s, int device) {
-    LsmBufs bb = b;
-    LsmSync ss = sy;
-    void* args[] = {&bb, &ss};
+hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStream_t s, int device) {
+    LsmPersistArgs a = pa;
+    void* args[] = {&a};
     // A plain launch: the grid fits the device's co-resident capacity (checked
     // by the caller), persistent launches run one after another on one stream
     // per device, and every other kernel of the library finishes on its own,
@@ -1490,30 +1531,30 @@ bool lds_fits_t(int B, int device) {
 // the basis size K: 3 (every mode), or 2, 4, 5, 6 (production arithmetic,
 // ref == false, checked by the caller).
 template <int PMAX, int K>
-hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid, int B,
+hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid, int B,
                                hipStream_t s, int dev) {
-    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(b, sy, grid, B, s, dev);
+    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(pa, grid, B, s, dev);
     if constexpr (K == 3) {
         if (src == kHost)
-            return ref ? launch_persistent_t<double, kHost, true, PMAX>(b, sy, grid, B, s, dev)
-                       : launch_persistent_t<double, kHost, false, PMAX>(b, sy, grid, B, s, dev);
-        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(b, sy, grid, B, s, dev)
-                   : launch_persistent_t<double, kPhilox, false, PMAX>(b, sy, grid, B, s, dev);
+            return ref ? launch_persistent_t<double, kHost, true, PMAX>(pa, grid, B, s, dev)
+                       : launch_persistent_t<double, kHost, false, PMAX>(pa, grid, B, s, dev);
+        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(pa, grid, B, s, dev)
+                   : launch_persistent_t<double, kPhilox, false, PMAX>(pa, grid, B, s, dev);
     } else {
         (void)ref;
-        return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(b, sy, grid, B, s, dev)
-                            : launch_persistent_t<double, kPhilox, false, PMAX, K>(b, sy, grid, B, s, dev);
+        return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(pa, grid, B, s, dev)
+                            : launch_persistent_t<double, kPhilox, false, PMAX, K>(pa, grid, B, s, dev);
     }
 }
 template <int PMAX>
-hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmBufs& b, const LsmSync& sy, int grid,
+hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid,
                              int B, hipStream_t s, int dev) {
     switch (K) {
-        case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, b, sy, grid, B, s, dev);
-        case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, b, sy, grid, B, s, dev);
-        case 5: return launch_persistent_k<PMAX, 5>(f64, src, ref, b, sy, grid, B, s, dev);
-        case 6: return launch_persistent_k<PMAX, 6>(f64, src, ref, b, sy, grid, B, s, dev);
-        default: return launch_persistent_k<PMAX, 3>(f64, src, ref, b, sy, grid, B, s, dev);
+        case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, dev);
+        case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, dev);
+        case 5: return launch_persistent_k<PMAX, 5>(f64, src, ref, pa, grid, B, s, dev);
+        case 6: return launch_persistent_k<PMAX, 6>(f64, src, ref, pa, grid, B, s, dev);
+        default: return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, dev);
     }
 }
 
@@ -1757,12 +1798,26 @@ void pcf_lsm_release(pcf_ctx* ctx) {
     graph_cache().erase(it);
 }
 
+namespace {
+int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T, int64_t N, int32_t M,
+                double payoff_fun, const pcf_opts& o, pcf_result* res);
+}  // namespace
+
 extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
                            int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts_in,
                            pcf_result* res) {
     if (!ctx || !res) return PCF_ERR_INVALID_ARG;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    const pcf_opts o = pcf_opts_or_default(opts_in);
+    pcf_request q = pcf_make_request(PCF_KIND_AMER, S0, E, r, sigma, T, N, payoff_fun, opts_in);
+    q.M = M;
+    const int st = mc_amer_run(ctx, S0, E, r, sigma, T, N, M, payoff_fun, q.opts, res);
+    if (st) return st;
+    return pcf_apply_fp64_check(ctx, q, res);
+}
+
+namespace {
+int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T, int64_t N, int32_t M,
+                double payoff_fun, const pcf_opts& o, pcf_result* res) {
     int st = pcf_check_common(ctx, S0, E, r, sigma, T, N, payoff_fun);
     if (st) return st;
     if ((st = pcf_check_opts(ctx, &o))) return st;
@@ -1816,13 +1871,15 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     const LsmShape sh = lsm_shape(N, mode, o, ctx->cu_count);
     const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
-    // coef (4 per date) | discount table | parameter block | sync words
-    // coef | disc | params | (256-byte aligned) persistent-form sync words
-    const size_t coef_bytes = sizeof(double) * (CS * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams) +
-                              512 + sizeof(unsigned int) * kSyncWords;
+    // persistent-form sync words (at a fixed place, zero at rest: zeroed when
+    // allocated, and every persistent launch leaves them zero) | coef (CS per
+    // date) | discount table | parameter block
+    const size_t sync_bytes = (sizeof(unsigned int) * kSyncWords + 255) / 256 * 256;
+    const size_t coef_bytes =
+        sync_bytes + sizeof(double) * (CS * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams);
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
     PCF_HIP_CHECK(ctx, ctx->lsm_state.ensure(state_bytes));
-    PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes));
+    PCF_HIP_CHECK(ctx, ctx->lsm_coef.ensure(coef_bytes, true));
 
     const int B = sh.B;
     const int grid_paths = sh.grid_paths;
@@ -1837,7 +1894,8 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     b.when = (int32_t*)sp;
     b.cf = sp + (size_t)sh.cap * sizeof(int32_t);
     b.st = sp + (size_t)sh.cap * (sizeof(int32_t) + rs);
-    double* cbase = ctx->lsm_coef.as<double>();
+    unsigned int* const sync_words = ctx->lsm_coef.as<unsigned int>();
+    double* cbase = (double*)(ctx->lsm_coef.as<char>() + sync_bytes);
     b.coef = cbase;
     b.disc = cbase + CS * (size_t)(M + 1);
     LsmParams* Pdev = (LsmParams*)(cbase + (CS + 1) * (size_t)(M + 1));
@@ -1845,29 +1903,6 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
     b.partials = ctx->partials.as<pcf::CSum>();
     b.counter = ctx->counters.as<unsigned int>();
     b.out = ctx->out.as<double>();
-
-    std::vector<double> host_blk((CS + 1) * (size_t)(M + 1), 0.0);
-    for (int k = 0; k <= M; ++k) host_blk[CS * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
-
-    if ((st = pcf_timing_begin(ctx))) return st;
-    if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
-    b.hw = src == kHost ? ctx->normals.as<double>() : nullptr;
-    // coef | disc | params are contiguous on the device: one copy, from the
-    // context's pinned staging buffer when it fits (the call synchronises
-    // before returning, so the staging is free again for the next call).
-    const size_t blk_bytes = host_blk.size() * sizeof(double);
-    if (blk_bytes + sizeof P <= kMaxBatch * kJobBytes) {
-        std::memcpy(ctx->h_jobs, host_blk.data(), blk_bytes);
-        std::memcpy((char*)ctx->h_jobs + blk_bytes, &P, sizeof P);
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, ctx->h_jobs, blk_bytes + sizeof P, hipMemcpyHostToDevice,
-                                          ctx->stream));
-    } else {
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, host_blk.data(), blk_bytes, hipMemcpyHostToDevice, ctx->stream));
-        PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
-    }
-
-    unsigned int* sync_words =
-        (unsigned int*)(((uintptr_t)((char*)Pdev + sizeof(LsmParams)) + 255) & ~(uintptr_t)255);
 
     // Persistent form when the request fits a co-resident grid.  Shape: the
     // tuned (block size, pairs per thread) for this size class, else explicit
@@ -1917,20 +1952,42 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
                             "persistent LSM form does not fit N=%lld (%lld blocks of %d, at most %d co-resident)",
                             (long long)N, (long long)g, bs, gmax);
     }
-    // Read-back of the per-date flags and the persistent form's fin/err words,
-    // staged in the context's pinned buffer (after out[0..3]) so the one
-    // synchronisation in pcf_timing_end covers every copy.
+    std::vector<double> host_blk((CS + 1) * (size_t)(M + 1), 0.0);
+    for (int k = 0; k <= M; ++k) host_blk[CS * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
+
+    if ((st = pcf_timing_begin(ctx))) return st;
+    if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
+    b.hw = src == kHost ? ctx->normals.as<double>() : nullptr;
+    // coef | disc | params are contiguous on the device: one copy, from the
+    // context's pinned staging buffer when it fits (the call synchronises
+    // before returning, so the staging is free again for the next call).  A
+    // persistent launch with M + 1 <= kDiscArgs takes them as kernel
+    // arguments instead and needs no copy.
+    const size_t blk_bytes = host_blk.size() * sizeof(double);
+    if (persistent && M + 1 <= kDiscArgs) {
+        // nothing to copy
+    } else if (blk_bytes + sizeof P <= kMaxBatch * kJobBytes) {
+        std::memcpy(ctx->h_jobs, host_blk.data(), blk_bytes);
+        std::memcpy((char*)ctx->h_jobs + blk_bytes, &P, sizeof P);
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, ctx->h_jobs, blk_bytes + sizeof P, hipMemcpyHostToDevice,
+                                          ctx->stream));
+    } else {
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(cbase, host_blk.data(), blk_bytes, hipMemcpyHostToDevice, ctx->stream));
+        PCF_HIP_CHECK(ctx, hipMemcpyAsync(Pdev, &P, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+    }
+
+
+    // Results.  The persistent form writes out[0..7] (kPersistOut) straight
+    // into the context's pinned buffer and clears its own sync words, so it
+    // needs no read-back copy.  The graph form's per-date flags are read back
+    // beside out[0..3] into the pinned buffer, so the one synchronisation in
+    // pcf_timing_end covers every copy.
     const size_t ncoef = CS * (size_t)(M + 1);
     const bool coef_pinned = 8 + ncoef <= (size_t)kOutDoubles;
-    unsigned int* const sync_pin = reinterpret_cast<unsigned int*>(ctx->h_out + 4);
-    sync_pin[0] = sync_pin[1] = sync_pin[2] = sync_pin[3] = 0u;
     auto stage_readback = [&]() -> int {
         if (coef_pinned)
             PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_out + 8, cbase, ncoef * sizeof(double), hipMemcpyDeviceToHost,
                                               ctx->stream));
-        if (persistent)
-            PCF_HIP_CHECK(ctx, hipMemcpyAsync(sync_pin, sync_words + kArrivalCounters * kCounterStride,
-                                              4 * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
         return PCF_OK;
     };
     if (persistent) {
@@ -1939,29 +1996,32 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)pgrid *
                                                 (2 * 8 * chunks * sizeof(double) + 4 * sizeof(pcf::CSum))));
         b.partials = ctx->partials.as<pcf::CSum>();
+        b.out = ctx->h_out_dev;
         const int fin_w = kArrivalCounters * kCounterStride;
-        LsmSync sy{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
-        PCF_HIP_CHECK(ctx, hipMemsetAsync(sync_words, 0, sizeof(unsigned int) * kSyncWords, ctx->stream));
+        LsmPersistArgs pa;
+        pa.b = b;
+        pa.sy = LsmSync{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
+        pa.P = P;
+        for (int k = 0; k <= M && k < kDiscArgs; ++k) pa.disc[k] = host_blk[CS * (size_t)(M + 1) + k];
         hipStream_t cs = nullptr;
         PCF_HIP_CHECK(ctx, coop_stream(ctx->device, &cs));
         // inputs ready on the context's stream -> the launch on the shared
-        // stream -> results read back on the context's stream
+        // stream -> completion waited on the context's stream
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ready, ctx->stream));
         {
             std::lock_guard<std::mutex> pl(persistent_mu());
             PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, cs));
             hipError_t le;
-            if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
-            else le = launch_persistent<8>(K, f64, src, ref, b, sy, pgrid, PB, cs, ctx->device);
+            if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
+            else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
+            else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
+            else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
             PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, cs));
         }
         PCF_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_kend, 0));
-        if ((st = stage_readback())) return st;
-        if ((st = pcf_timing_end(ctx, 4, res))) return st;
+        if ((st = pcf_timing_end(ctx, 0, res))) return st;
         res->grid_blocks = pgrid;
         res->block_size = PB;
     }
@@ -2024,12 +2084,20 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         res->block_size = B;
     }
     const double h0 = ctx->h_out[0], h1 = ctx->h_out[1], h2 = ctx->h_out[2], h3 = ctx->h_out[3];
+    int small = 0, fallback = 0, singular = 0;
+    if (persistent) {
+        small = (int)ctx->h_out[4];
+        fallback = (int)ctx->h_out[5];
+        singular = (int)ctx->h_out[6];
+        if (ctx->h_out[7] != 0.0) return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
+    }
 
-    // Per-date flags (beyond the pinned staging area: one more copy).
+    // Per-date flags of the graph form beyond the pinned staging area, and the
+    // optional exercise dates: one more copy each.
     std::vector<double> coefs;
     const double* cf_host = ctx->h_out + 8;
-    if (!coef_pinned || o.when_out) {
-        if (!coef_pinned) {
+    if ((!persistent && !coef_pinned) || o.when_out) {
+        if (!persistent && !coef_pinned) {
             coefs.resize(ncoef);
             PCF_HIP_CHECK(ctx, hipMemcpyAsync(coefs.data(), cbase, ncoef * sizeof(double), hipMemcpyDeviceToHost,
                                               ctx->stream));
@@ -2040,15 +2108,13 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
                                               hipMemcpyDeviceToHost, ctx->stream));
         PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    if (persistent && sync_pin[1])
-        return pcf_fail(ctx, PCF_ERR_TIMEOUT, "LSM date barrier wait exceeded its bound");
-    int small = 0, fallback = 0, singular = 0;
-    for (int m = 1; m <= M - 1; ++m) {
-        const int f = (int)cf_host[CS * m];
-        small += f == kFlagSmall;
-        fallback += f == kFlagReduced;
-        singular += f == kFlagSingular;
-    }
+    if (!persistent)
+        for (int m = 1; m <= M - 1; ++m) {
+            const int f = (int)cf_host[CS * m];
+            small += f == kFlagSmall;
+            fallback += f == kFlagReduced;
+            singular += f == kFlagSingular;
+        }
 
     const double n = (double)N;
     const double pay0 = std::max(payoff_fun * (S0 - E), 0.0);
@@ -2074,3 +2140,4 @@ extern "C" int pcf_mc_amer(pcf_ctx* ctx, double S0, double E, double r, double s
         return pcf_fail(ctx, PCF_ERR_SINGULAR, "Detereminant is not > 0 (%d dates)", singular);
     return PCF_OK;
 }
+}  // namespace

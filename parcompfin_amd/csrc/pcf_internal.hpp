@@ -62,6 +62,9 @@ constexpr int kOutPerJob = 8;
 constexpr int kOutDoubles = kMaxBatch * kOutPerJob;
 // Bytes of one job record (launch descriptor) staged per batched launch.
 constexpr size_t kJobBytes = 512;
+// Device-resident basket constants: kMaxBatch slots of 64 KB per context.
+constexpr size_t kBasketConstSlot = 64 * 1024;
+constexpr int64_t kBasketConstBytes = (int64_t)kBasketConstSlot * kMaxBatch;
 
 // Device-resident basket constants (factor + per-asset terms), LRU slots.
 struct BasketCache {
@@ -89,6 +92,8 @@ struct pcf_ctx {
     int64_t logtab_n = -1;
     DevBuf prefix;     // binom: prefix chain of the descending log sum
     int64_t prefix_n = -1;
+    DevBuf logfact;    // binom production: log k! as double-double, k = 0..logfact_n
+    int64_t logfact_n = -1;
     DevBuf lsm_store;  // mc_amer path store
     DevBuf lsm_state;  // mc_amer per-path state
     DevBuf lsm_coef;   // mc_amer per-date coefficients / flags
@@ -100,6 +105,7 @@ struct pcf_ctx {
     void* lattice_ptr = nullptr;          // where they were written
     BasketCache basket_cache;
     double* h_out = nullptr;  // pinned mirror of `out`
+    double* h_out_dev = nullptr;  // h_out as kernels address it (persistent LSM results)
     void* h_jobs = nullptr;   // pinned staging of `jobs`
 };
 
@@ -198,6 +204,11 @@ uint64_t pcf_basket_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
 int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
 // Device bytes an American request holds while in flight (path store + state).
 int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact);
+
+// opts.fp64_check on a PCF_FP32K Philox request of a Monte Carlo kind: prices
+// q again in PCF_FP64 on the same Philox counters and stores |price - FP64
+// price| in res->fp64_err (res->fp64_checked = 1).  A no-op otherwise.
+int pcf_apply_fp64_check(pcf_ctx* ctx, const pcf_request& q, pcf_result* res);
 
 // glibc pow, never folded by the compiler.  Verification constants follow
 // the reference's -O0 build, which calls glibc pow(x, 2.0); compilers fold

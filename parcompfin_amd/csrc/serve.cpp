@@ -5,20 +5,23 @@
 // partials, counters, the LSM path store and its captured graphs, cached
 // basket factors).  Work issued by different slots overlaps on the device.
 //
-// Admission: a request's device footprint (the full path store for the
-// American, reduction partials otherwise) is charged against mem_budget_bytes
-// while it is in flight.  A free slot takes the oldest queued request that
-// fits the remaining budget (smaller requests may pass one that does not fit
-// yet) and, for the single-launch kinds (European, Asian, basket), every
-// further queued request with the same batch key -- kernel variant and
-// launch shape -- up to kMaxBatch, and prices them in one batched launch.
-// Requests that arrive while others run join the next launch of a free slot.
+// Memory: every slot is charged for the request-sized device buffers it may
+// hold -- path stores and per-path state, reduction partials, binomial
+// tables -- and the charges of all slots never exceed mem_budget_bytes.  An
+// idle slot is charged what it keeps from its last request (measured); a
+// busy slot what it kept plus the footprint of the unit it runs (the full
+// path store for the American, the tables for the binomial, partials
+// otherwise), an upper bound of what the unit can add.  A free slot takes
+// the oldest queued request whose charge fits (smaller requests may pass one
+// that does not fit yet), first by dropping its own kept buffers, then by
+// freeing idle slots' kept buffers, least recently used first; it also takes
+// every further queued request with the same batch key (single-launch kinds
+// only: European, Asian, basket) up to kMaxBatch, and prices them in one
+// batched launch.  Requests that arrive while others run join the next
+// launch of a free slot.  The per-slot buffers that do not grow with a
+// request (job table, arrival counters, results, basket factor slots) are
+// outside the budget; pcf_server_stats reports them.
 //
-// Results do not depend on the mix: a batched launch reduces each request
-// over exactly the blocks and per-thread paths its lone launch would use
-// (shapes come from the process-wide tuned table), and the Philox stream is
-// keyed by (seed, path, step).  The American runs its captured per-class
-// graph on the slot; the binomial sum runs alone on its slot.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -59,11 +62,15 @@ struct pcf_server {
     std::deque<Item> queue;
     std::unordered_map<uint64_t, Finished> done;
     std::unordered_map<uint64_t, int> live;  // submitted, not yet collected
-    int64_t budget = 0, in_use = 0;
-    // Path-store bytes that slots keep between American requests (per slot,
-    // and their total), bounded by the budget as a whole.
-    std::vector<int64_t> resident;
-    int64_t resident_total = 0;
+    int64_t budget = 0;
+    // Per slot: request-sized bytes it holds (measured when it went idle),
+    // its charge against the budget, busy flag and last use (for eviction).
+    std::vector<int64_t> held, charge;
+    std::vector<char> busy;
+    std::vector<uint64_t> last_used;
+    uint64_t use_clock = 0;
+    int64_t charged = 0;  // sum of charge
+    pcf_server_stats st{};
     bool stop = false;
     uint64_t next_ticket = 1;
     clk::time_point t0;
@@ -82,10 +89,42 @@ int64_t footprint_of(const pcf_request& q, int64_t nblk_hint) {
         case PCF_KIND_AMER:
             return pcf_amer_footprint(q.N, q.M, q.opts.precision, q.opts.rng, q.opts.ref_exact);
         case PCF_KIND_BINOM:
-            return (q.N + 1) * 8 + ((q.N / 2 + 256) / 256 + 1) * 16 + 4096;
+            // chain: glibc log table + prefix table, (N+1) doubles each; double-
+            // double: log k! table, (N+1) x 16 B; both: one partial per 256 terms
+            return (q.opts.ref_exact ? 2 * (q.N + 1) * 8 : (q.N + 1) * 16) + ((q.N + 256) / 256 + 1) * 16 + 4096;
         default:
             return nblk_hint * 4 * 16 + 4096;
     }
+}
+
+// Request-sized device bytes a context holds (the buffers that grow with the
+// requests it runs).
+int64_t request_bytes(pcf_ctx* ctx) {
+    int64_t b = 0;
+    for (const DevBuf* d : {&ctx->partials, &ctx->normals, &ctx->logtab, &ctx->prefix, &ctx->logfact,
+                            &ctx->lsm_store, &ctx->lsm_state, &ctx->lsm_coef, &ctx->scratch, &ctx->lattice})
+        b += (int64_t)d->bytes;
+    return b;
+}
+
+// Frees a context's request-sized buffers (the context must be idle).
+void release_request_buffers(pcf_ctx* ctx) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    pcf_lsm_release(ctx);
+    for (DevBuf* d : {&ctx->partials, &ctx->normals, &ctx->logtab, &ctx->prefix, &ctx->logfact, &ctx->lsm_store,
+                      &ctx->lsm_state, &ctx->lsm_coef, &ctx->scratch, &ctx->lattice})
+        d->release();
+    ctx->logtab_n = ctx->prefix_n = ctx->logfact_n = -1;
+    ctx->lattice_key[0] = ctx->lattice_key[1] = ctx->lattice_key[2] = 0;
+    ctx->lattice_ptr = nullptr;
+}
+
+void set_charge(pcf_server* srv, int slot, int64_t c) {
+    srv->charged += c - srv->charge[slot];
+    srv->charge[slot] = c;
+    if (srv->charged > srv->st.peak_charged_bytes) srv->st.peak_charged_bytes = srv->charged;
 }
 
 // Runs one unit (items sharing a batch key, or one item) on a slot context.
@@ -103,6 +142,11 @@ void run_unit(pcf_server* srv, int slot, pcf_ctx* ctx, std::vector<Item>& unit, 
         if (kind == PCF_KIND_EUR) st = pcf_eur_run(ctx, n, reqs.data(), res.data(), status.data());
         else if (kind == PCF_KIND_ASIA) st = pcf_asia_run(ctx, n, reqs.data(), res.data(), status.data());
         else if (kind == PCF_KIND_BASKET) st = pcf_basket_run(ctx, n, reqs.data(), res.data(), status.data());
+        // batched requests asking for the FP64 comparison get their FP64 run here
+        // (an American request does it inside pcf_mc_amer)
+        if (st == PCF_OK && kind != PCF_KIND_AMER && kind != PCF_KIND_BINOM)
+            for (int i = 0; i < n; ++i)
+                if (status[i] == PCF_OK) status[i] = pcf_apply_fp64_check(ctx, unit[i].req, &res[i]);
     }
     if (kind == PCF_KIND_AMER || kind == PCF_KIND_BINOM) {
         for (int i = 0; i < n; ++i) {
@@ -110,28 +154,6 @@ void run_unit(pcf_server* srv, int slot, pcf_ctx* ctx, std::vector<Item>& unit, 
             status[i] = kind == PCF_KIND_AMER
                             ? pcf_mc_amer(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.M, q.payoff_fun, &q.opts, &res[i])
                             : pcf_binom_embar(ctx, q.S0, q.E, q.r, q.sigma, q.T, q.N, q.payoff_fun, &q.opts, &res[i]);
-        }
-        // A slot keeps its path store for the next American request unless
-        // the stores kept by all slots would exceed the budget; then this one
-        // is freed.  A fixed share per slot (budget / slots) would instead
-        // free and reallocate a store on every request once that share is
-        // smaller than one store.
-        if (kind == PCF_KIND_AMER) {
-            const int64_t held = (int64_t)(ctx->lsm_store.bytes + ctx->lsm_state.bytes);
-            bool release;
-            {
-                std::lock_guard<std::mutex> lk(srv->mu);
-                srv->resident_total += held - srv->resident[slot];
-                release = srv->resident_total > srv->budget;
-                if (release) srv->resident_total -= held;
-                srv->resident[slot] = release ? 0 : held;
-            }
-            if (release) {
-                std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-                pcf_lsm_release(ctx);
-                ctx->lsm_store.release();
-                ctx->lsm_state.release();
-            }
         }
     }
     const double t = srv->now_ms();
@@ -151,20 +173,52 @@ void run_unit(pcf_server* srv, int slot, pcf_ctx* ctx, std::vector<Item>& unit, 
     }
 }
 
-// Picks the next unit under srv->mu; returns false if nothing is admissible.
-bool take_unit(pcf_server* srv, std::vector<Item>& unit) {
+// Picks the next unit for `slot` under srv->mu and charges it; returns false
+// if nothing is admissible.  *drop_own: the slot must free its own kept
+// buffers before running the unit (its charge no longer includes them).
+// Idle slots' kept buffers freed here to make room are freed at once.
+bool take_unit(pcf_server* srv, int slot, std::vector<Item>& unit, bool* drop_own) {
     unit.clear();
+    *drop_own = false;
     auto& q = srv->queue;
+    const int64_t others = srv->charged - srv->charge[slot];
+    const int64_t own = srv->held[slot];
+    // Kept buffers of other idle slots that may be freed, least recently used first.
+    std::vector<int> idle;
+    int64_t evictable = 0;
+    for (int s = 0; s < (int)srv->slots.size(); ++s)
+        if (s != slot && !srv->busy[s] && srv->held[s] > 0) {
+            idle.push_back(s);
+            evictable += srv->held[s];
+        }
     for (size_t i = 0; i < q.size(); ++i) {
-        if (q[i].footprint > srv->budget - srv->in_use) continue;
+        const int64_t fp = q[i].footprint;
+        bool drop = false;
+        if (others + own + fp > srv->budget) {
+            drop = own > 0;
+            if (others + fp > srv->budget && others - evictable + fp > srv->budget) continue;
+        }
+        int64_t room = srv->budget - others - (drop ? 0 : own);
+        if (fp > room) {
+            std::sort(idle.begin(), idle.end(),
+                      [&](int a, int b) { return srv->last_used[a] < srv->last_used[b]; });
+            for (int s : idle) {
+                if (fp <= room) break;
+                release_request_buffers(srv->slots[s]);
+                room += srv->held[s];
+                srv->held[s] = 0;
+                set_charge(srv, s, 0);
+                ++srv->st.evictions;
+            }
+        }
         Item head = q[i];
         q.erase(q.begin() + (long)i);
-        int64_t fp = head.footprint;
+        int64_t total = head.footprint;
         unit.push_back(head);
         if (head.key != 0) {
             for (size_t j = i; j < q.size() && (int)unit.size() < kMaxBatch;) {
-                if (q[j].key == head.key && fp + q[j].footprint <= srv->budget - srv->in_use) {
-                    fp += q[j].footprint;
+                if (q[j].key == head.key && total + q[j].footprint <= room) {
+                    total += q[j].footprint;
                     unit.push_back(q[j]);
                     q.erase(q.begin() + (long)j);
                 } else {
@@ -172,7 +226,13 @@ bool take_unit(pcf_server* srv, std::vector<Item>& unit) {
                 }
             }
         }
-        srv->in_use += fp;
+        *drop_own = drop;
+        if (drop) {
+            srv->held[slot] = 0;
+            ++srv->st.evictions;
+        }
+        srv->busy[slot] = 1;
+        set_charge(srv, slot, (drop ? 0 : own) + total);
         const double t = srv->now_ms();
         for (auto& it : unit) it.start_ms = t;
         return true;
@@ -186,21 +246,49 @@ void worker_main(pcf_server* srv, int slot) {
     std::vector<Item> unit;
     std::vector<Finished> fin;
     for (;;) {
+        bool drop_own = false;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             // Queued work is drained before a stop takes effect.
             srv->cv_work.wait(lk, [&] {
-                return take_unit(srv, unit) || (srv->stop && srv->queue.empty());
+                return take_unit(srv, slot, unit, &drop_own) || (srv->stop && srv->queue.empty());
             });
             if (unit.empty()) return;
         }
+        if (drop_own) release_request_buffers(ctx);
         run_unit(srv, slot, ctx, unit, fin);
+        int64_t now_held;
+        {
+            std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+            now_held = request_bytes(ctx);
+        }
+        bool over = false;
         {
             std::lock_guard<std::mutex> lk(srv->mu);
-            for (size_t i = 0; i < unit.size(); ++i) {
-                srv->in_use -= unit[i].footprint;
-                srv->done[unit[i].ticket] = fin[i];
+            // The charge bounded what the unit could hold; anything above it
+            // is an estimate error, counted and then freed.
+            if (now_held > srv->charge[slot]) {
+                ++srv->st.underestimates;
+                over = true;
             }
+            for (size_t i = 0; i < unit.size(); ++i) srv->done[unit[i].ticket] = fin[i];
+            srv->st.completed += (int64_t)unit.size();
+        }
+        if (over) {
+            release_request_buffers(ctx);
+            now_held = 0;
+        }
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->held[slot] = now_held;
+            set_charge(srv, slot, now_held);
+            srv->busy[slot] = 0;
+            srv->last_used[slot] = ++srv->use_clock;
+            int64_t sum = 0;
+            for (int64_t h : srv->held) sum += h;
+            // busy slots count what they held before their unit: a lower
+            // bound of their current use; the charge is the upper bound
+            if (sum > srv->st.peak_held_bytes) srv->st.peak_held_bytes = sum;
         }
         srv->cv_done.notify_all();
         srv->cv_work.notify_all();  // budget freed
@@ -219,7 +307,12 @@ extern "C" int pcf_server_create(pcf_ctx* ctx, const pcf_server_config* cfg, pcf
     auto* srv = new pcf_server();
     srv->device = ctx->device;
     srv->budget = c.mem_budget_bytes > 0 ? c.mem_budget_bytes : ((int64_t)8 << 30);
-    srv->resident.assign(nslots, 0);
+    srv->held.assign(nslots, 0);
+    srv->charge.assign(nslots, 0);
+    srv->busy.assign(nslots, 0);
+    srv->last_used.assign(nslots, 0);
+    srv->st.budget_bytes = srv->budget;
+    srv->st.slots = nslots;
     srv->t0 = clk::now();
     int st = pcf_ctx_create(ctx->device, &srv->front);
     if (st != PCF_OK) {
@@ -334,4 +427,18 @@ extern "C" const char* pcf_server_last_error(pcf_server* srv) {
     if (!srv) return "null server";
     std::lock_guard<std::mutex> lk(srv->mu);
     return srv->last_error.c_str();
+}
+
+extern "C" int pcf_server_get_stats(pcf_server* srv, pcf_server_stats* out) {
+    if (!srv || !out) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *out = srv->st;
+    out->charged_bytes = srv->charged;
+    int64_t h = 0;
+    for (int64_t v : srv->held) h += v;
+    out->held_bytes = h;
+    // arrival counters, result slots, job table, basket factor slots
+    out->fixed_bytes_per_slot = (int64_t)(kCounterSlots * sizeof(unsigned int) + kOutDoubles * sizeof(double) +
+                                          kMaxBatch * kJobBytes) + kBasketConstBytes;
+    return PCF_OK;
 }

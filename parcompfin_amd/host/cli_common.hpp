@@ -16,7 +16,8 @@
 //                       device (reproduces the reference program's result)
 //     --ref-exact       reference arithmetic choices (binom comb chain -- default for
 //                       binom_embar; LSM quirks; basket without sqrt(T))
-//     --production      binom_embar: FP64 lgamma coefficients instead of the chain
+//     --production      binom_embar: double-double log-space coefficients (within
+//                       ~1e-15 of the exact sum) instead of the reference's chain
 //     --comparison X    value subtracted in the Abs_Error/Error columns (the
 //                       reference compiles it in via include/comparison.h; default 0;
 //                       the PCF_COMPARISON environment variable also works)
@@ -25,7 +26,9 @@
 //                       rebuild, runscript_mc_eur.sh:10-11); also PCF_COMPARISON_FILE
 //     --header          print the runscripts' CSV header line before the row
 //                       (runscript_mc_eur.sh:13)
-//     --stats           one extra line on stderr: SE, kernel time, control values
+//     --stats           one extra line on stderr: SE, kernel time, control values and,
+//                       for an FP32/Kahan run, |FP32/Kahan - FP64| on the same Philox
+//                       paths (the FP64 run is made for it)
 // A trailing plain integer (the _omp programs' thread count) is accepted and ignored.
 // Errors: message on stderr, exit status 1 (the reference aborts with an exception).
 #pragma once
@@ -230,10 +233,14 @@ inline void stats_line(const Args& a, const pcf_result& r) {
     if (!a.stats) return;
     std::fprintf(stderr,
                  "stats: price=%.17g se=%.6g kernel_ms=%.6f total_ms=%.6f control=%.10g control_se=%.3g "
-                 "paths=%lld steps=%lld small_itm=%d fallback=%d grid=%d block=%d seed=%llu\n",
+                 "paths=%lld steps=%lld small_itm=%d fallback=%d grid=%d block=%d seed=%llu",
                  r.price, r.std_err, r.kernel_ms, r.total_ms, r.control_price, r.control_se,
                  (long long)r.n_paths, (long long)r.n_steps, r.lsm_small_itm, r.lsm_fallback,
                  r.grid_blocks, r.block_size, a.seed);
+    if (r.fp64_checked)
+        std::fprintf(stderr, " fp64_err=%.6g fp64_err_se=%.4g", r.fp64_err,
+                     r.std_err > 0 ? r.fp64_err / r.std_err : 0.0);
+    std::fprintf(stderr, "\n");
 }
 
 inline pcf_opts base_opts(const Args& a) {
@@ -242,6 +249,9 @@ inline pcf_opts base_opts(const Args& a) {
     o.seed = a.seed;
     o.precision = a.precision;
     o.ref_exact = a.ref_exact ? 1 : 0;
+    // --stats on an FP32/Kahan run also prices the same paths in FP64 and
+    // reports the difference
+    o.fp64_check = a.stats ? 1 : 0;
     return o;
 }
 

@@ -26,9 +26,12 @@ def run(prog, *args, env=None):
 
 
 PROGS = ["mc_eur", "binom_embar", "mc_asia", "mc_amer", "mc_eur_multi", "binom_vanilla_eur", "binom_vanilla_amer"]
+# the names the runscripts' OpenMP sweeps call (runscript_mc_eur.sh:35, runscript_mc_asia.sh:43,
+# runscript_mc_amer.sh:41, runscript_binom_embar.sh:37; src/mc_eur_multi_omp.cpp)
+OMP_PROGS = ["mc_eur_omp", "binom_embar_omp", "mc_asia_omp", "mc_amer_omp", "mc_eur_multi_omp"]
 
 
-@pytest.mark.parametrize("prog", PROGS)
+@pytest.mark.parametrize("prog", PROGS + OMP_PROGS)
 def test_programs_built(prog):
     assert os.access(os.path.join(BIN, prog), os.X_OK), "run __graft_entry__.build() first"
 
@@ -161,9 +164,41 @@ def test_amer_basis_flag():
 
 
 @pytest.mark.gpu
+def test_stats_line_states_the_fp64_error():
+    p = run("mc_asia", "call", 100, 100, 0.05, 0.25, 1, 1 << 16, 252, "--seed", 3, "--stats")
+    assert p.returncode == 0, p.stderr
+    line = [l for l in p.stderr.splitlines() if l.startswith("stats:")][0]
+    kv = dict(t.split("=", 1) for t in line.split()[1:])
+    assert float(kv["fp64_err_se"]) < 0.05 and float(kv["fp64_err"]) >= 0.0
+    q = run("mc_asia", "call", 100, 100, 0.05, 0.25, 1, 1 << 16, 252, "--seed", 3, "--precision", "fp64")
+    d = abs(float(parse_row(p.stdout)["Result"]) - float(parse_row(q.stdout)["Result"]))
+    assert abs(d - float(kv["fp64_err"])) < 1e-8  # the rows print 10 significant digits
+
+
+@pytest.mark.gpu
 def test_trailing_thread_count_is_accepted():
     # the _omp programs' argv (src/mc_eur_omp.cpp:41): a trailing thread count
     p = run("mc_eur", "call", 100, 100, 0.05, 0.2, 1, 100000, 16)
     assert p.returncode == 0, p.stderr
     got = parse_row(p.stdout)
     assert got["Method"] == "HIP" and abs(float(got["Result"]) - 10.45) < 0.2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog", OMP_PROGS)
+def test_omp_names_print_the_same_row(prog):
+    """Each <prog>_omp takes the reference OpenMP program's argv, trailing
+    thread count included, and prints the row of <prog> for the same input."""
+    base = prog[:-len("_omp")]
+    pos = {"mc_eur": [100000], "binom_embar": [2000], "mc_asia": [4096, 50], "mc_amer": [4096, 20],
+           "mc_eur_multi": [1 << 14, 8, 0.3]}[base]
+    args = ["call", 100, 100, 0.05, 0.2, 1] + pos
+    flags = [] if base == "binom_embar" else ["--seed", 7]
+    print("[case]", prog, *args, flush=True)
+    p = run(prog, *args, 16, *flags)
+    q = run(base, *args, *flags)
+    assert p.returncode == 0 and q.returncode == 0, (p.stderr, q.stderr)
+    a, b = parse_row(p.stdout), parse_row(q.stdout)
+    for c in COLUMNS:
+        if c not in ("T_overall", "T_calculation"):
+            assert a[c] == b[c], (prog, c, a[c], b[c])

@@ -139,6 +139,75 @@ def test_basket_perfect_correlation_is_black_scholes(engine):
     assert abs(r.price - bs) < 4 * r.std_err
 
 
+@pytest.mark.parametrize("T", [0.5, 2.0])
+def test_basket_production_sqrt_T(engine, T):
+    """Production (ref_exact 0) at T != 1, where the reference's missing
+    sqrt(T) matters:
+      * d = 1, FP64 Philox: the reference's mc_eur (src/mc_eur.cpp:15-27) on
+        the same normals times sqrt(T), to 1e-12;
+      * d = 8, FP64 Philox, two correlations: the oracle's production basket on
+        the same normals, to 1e-12;
+      * host normals (production arithmetic, reference order): the oracle's
+        production basket to 1e-12;
+      * rho = 1, FP64: the single-asset Black-Scholes price within 4 SE."""
+    say("basket production sqrt(T)", T)
+    N = 1 << 14
+    z = np.empty(N)
+    O.lib().orc_philox_basket_normals(31, N, 1, O.ptr(z))
+    want = O.lib().orc_mc_eur_z(100, 100, 0.05, 0.2, T, N, 1.0, O.ptr(z * math.sqrt(T)))
+    got = engine.mc_eur_multi(100, 100, 0.05, 0.2, T, N, "call", 1, 0.0, seed=31, precision="fp64")
+    assert rel(got.price, want) < REL
+    d = 8
+    Z = np.empty(N * d)
+    O.lib().orc_philox_basket_normals(32, N, d, O.ptr(Z))
+    sig = np.random.default_rng(3).uniform(0.15, 0.35, size=d)
+    for corr in (equicorr(d, 0.3), random_corr(d, 5)):
+        L = chol(corr)
+        for pf in ("call", "put"):
+            p = 1.0 if pf == "call" else -1.0
+            want = O.lib().orc_basket_prod_z(100, 100, 0.05, 0.2, T, N, p, d, O.ptr(L), O.ptr(Z), O.ptr(sig),
+                                             None, None, None)
+            got = engine.mc_eur_multi(100, 100, 0.05, 0.2, T, N, pf, d, 0.3, sigmas=sig, corr=corr, seed=32,
+                                      precision="fp64")
+            assert rel(got.price, want) < REL
+            goth = engine.mc_eur_multi(100, 100, 0.05, 0.2, T, N, pf, d, 0.3, sigmas=sig, corr=corr, normals=Z)
+            assert rel(goth.price, want) < REL
+    r = engine.mc_eur_multi(100, 100, 0.05, 0.2, T, 1 << 20, "put", 4, 1.0, seed=6, precision="fp64")
+    bs = O.lib().orc_bs_price(100, 100, 0.05, 0.2, T, -1.0)
+    assert abs(r.price - bs) < 4 * r.std_err
+
+
+def test_basket_factor_cache_collisions_never_mix_baskets(engine):
+    """With the cache hash masked to a single value every basket collides;
+    each price still equals its price with the full hash (the cache compares
+    the full key on a hit), alone and interleaved, and in a served batch."""
+    cases = [dict(assets=8, rho=0.3), dict(assets=8, rho=0.5), dict(assets=8, rho=0.3, sigmas=np.full(8, 0.25)),
+             dict(assets=4, rho=0.3), dict(assets=8, rho=0.3, corr=random_corr(8, 2))]
+    N = 1 << 14
+
+    def price(kw, s):
+        return engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", seed=s, precision="fp64", **kw).price
+
+    from parcompfin_amd import _native
+    L = _native.load()
+    say("basket cache, full hash")
+    want = [price(kw, 40 + i) for i, kw in enumerate(cases)]
+    try:
+        L.pcf_debug_basket_hash_mask(0)
+        say("basket cache, every hash equal")
+        for rnd in range(2):
+            for i in ([0, 1, 2, 3, 4] if rnd == 0 else [4, 2, 0, 3, 1]):
+                assert price(cases[i], 40 + i) == want[i], (rnd, i)
+        from parcompfin_amd import Server
+        with Server(engine, max_inflight=1) as srv:
+            ts = [srv.submit("basket", 100, 100, 0.05, 0.2, 1.0, N, "call", seed=40 + i, precision="fp64", **kw)
+                  for i, kw in enumerate(cases)]
+            got = [srv.wait(t).price for t in ts]
+        assert got == want
+    finally:
+        L.pcf_debug_basket_hash_mask(0xFFFFFFFFFFFFFFFF)
+
+
 def test_basket_invalid(engine):
     from parcompfin_amd import PcfError
     say("basket invalid")

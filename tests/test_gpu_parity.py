@@ -161,6 +161,27 @@ def test_asia_verification_vs_reference_fixture(engine):
         assert rel(got.price, float(r["result"])) < REL
 
 
+@pytest.mark.parametrize("S0,E,r,s,T,M,pf", [(100, 100, 0.05, 0.25, 1.0, 252, "call"),
+                                             (100, 110, 0.02, 0.75, 2.0, 50, "call"),
+                                             (90, 100, 0.08, 0.3, 0.5, 7, "put")])
+def test_asia_production_arithmetic_on_reference_draws(engine, S0, E, r, s, T, M, pf):
+    """The production FP64 step (kernel_variant 1 on host draws) prices the
+    reference's own mt19937 draws as the reference does (src/mc_asia.cpp:27-37)
+    and as the oracle's production restatement does, to 1e-12."""
+    say("mc_asia production arithmetic on host draws", M, pf, T)
+    N = 2048
+    w = O.normals_mt(23, math.sqrt(T / M), N * M)
+    p = 1.0 if pf == "call" else -1.0
+    ref = O.lib().orc_mc_asia_z(S0, E, r, s, T, N, M, p, O.ptr(w))
+    prod = O.lib().orc_mc_asia_prod_z(S0, E, r, s, T, N, M, p, O.ptr(w), None)
+    got = engine.mc_asia(S0, E, r, s, T, N, M, pf, normals=w, kernel_variant=1)
+    assert rel(got.price, prod) < REL
+    assert rel(got.price, ref) < REL
+    # kernel_variant 0 on the same draws is the reference's own arithmetic
+    got0 = engine.mc_asia(S0, E, r, s, T, N, M, pf, normals=w)
+    assert rel(got0.price, ref) < REL
+
+
 @pytest.mark.parametrize("M", [252, 13])
 def test_asia_fp64_philox_vs_oracle(engine, M):
     say("mc_asia fp64 philox vs oracle", M)
@@ -196,7 +217,8 @@ def test_binom_chain_vs_oracle(engine, N):
 def test_binom_chain_vs_reference_fixtures(engine):
     rows = [r for r in csv.DictReader(open(os.path.join(HERE, "golden", "ref_kernels.csv")))
             if r["prog"] == "binom_embar"]
-    assert any(int(r["N"]) >= 100000 for r in rows) or True
+    # the reference driver's own N = 1e5 value is among them
+    assert any(int(r["N"]) == 100000 for r in rows)
     for r in rows:
         N = int(r["N"])
         say("binom fixture", N, r["payoff"])
@@ -205,11 +227,39 @@ def test_binom_chain_vs_reference_fixtures(engine):
         assert rel(got.price, float(r["result"])) < REL
 
 
-def test_binom_lgamma_production(engine):
-    for N in (1000, 100000):
-        say("binom lgamma", N)
-        chain = engine.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True)
-        lg = engine.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False)
-        assert rel(lg.price, chain.price) < 1e-8
-    bs = O.lib().orc_bs_price(100, 100, 0.05, 0.2, 1.0, 1.0)
-    assert abs(lg.price - bs) < 1e-4
+def _csv(name):
+    with open(os.path.join(HERE, "golden", name)) as f:
+        return list(csv.DictReader(f))
+
+
+@pytest.mark.parametrize("row", _csv("binom_large.csv"), ids=lambda r: f"{r['payoff']}-N{r['N']}")
+def test_binom_chain_at_1e5_and_1e6(engine, row):
+    """The reference-exact chain at N = 1e5 and 1e6 (call and put) against the
+    prefix form's values, which equal the reference's chain bit for bit."""
+    N = int(row["N"])
+    say("binom chain large", N, row["payoff"])
+    got = engine.binom_embar(100, 100, 0.05, 0.2, 1.0, N, row["payoff"], ref_exact=True)
+    assert rel(got.price, float(row["result"])) < REL
+
+
+@pytest.mark.parametrize("row", _csv("binom_exact.csv"), ids=lambda r: f"{r['payoff']}-N{r['N']}-E{r['E']}")
+def test_binom_dd_production_vs_exact_sum(engine, row):
+    """The double-double production form against the exact sum (mpmath, 40
+    digits) to 1e-13, and against the oracle's restatement to 1e-13."""
+    N = int(row["N"])
+    S0, E, r, s, T = (float(row[k]) for k in ("S0", "E", "r", "sigma", "T"))
+    say("binom dd", N, row["payoff"], E)
+    got = engine.binom_embar(S0, E, r, s, T, N, row["payoff"], ref_exact=False)
+    assert rel(got.price, float(row["exact"])) < 1e-13
+    pf = 1.0 if row["payoff"] == "call" else -1.0
+    assert rel(got.price, O.lib().orc_binom_dd(S0, E, r, s, T, N, pf)) < 1e-13
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 100, 101, 4097, 30000])
+@pytest.mark.parametrize("pf", ["call", "put"])
+def test_binom_dd_production_vs_oracle(engine, N, pf):
+    say("binom dd vs oracle", N, pf)
+    p = 1.0 if pf == "call" else -1.0
+    want = O.lib().orc_binom_dd(100, 110, 0.02, 0.75, 1.0, N, p)
+    got = engine.binom_embar(100, 110, 0.02, 0.75, 1.0, N, pf, ref_exact=False)
+    assert rel(got.price, want) < 1e-13

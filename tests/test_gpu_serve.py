@@ -210,6 +210,39 @@ def test_autotune_then_invariance(engine):
         tune_reset()
 
 
+def test_fp64_error_per_result(engine):
+    """opts.fp64_check: every FP32/Kahan result carries |FP32/Kahan - FP64| on
+    the same Philox paths, direct and served; it equals the difference of the
+    two lone calls exactly, and it is a small fraction of a standard error."""
+    reqs = mixed_requests(17, 10, 14, 17)
+    say("fp64 check direct")
+    for q in reqs:
+        a = price_alone_kw(engine, q, fp64_check=True)
+        b = price_alone(engine, q, precision="fp64")
+        assert a.fp64_checked == 1 and a.fp64_err == abs(a.price - b.price), q["kind"]
+        assert a.fp64_err < 0.05 * b.std_err
+        c = price_alone(engine, q)
+        assert c.fp64_checked == 0 and c.price == a.price
+    say("fp64 check served")
+    with Server(engine, max_inflight=3) as srv:
+        ts = [srv.submit(q["kind"], *q["args"], seed=q["seed"], fp64_check=True, **q["kw"]) for q in reqs]
+        served = [srv.wait(t) for t in ts]
+    for q, s in zip(reqs, served):
+        b = price_alone(engine, q, precision="fp64")
+        assert s.fp64_checked == 1 and s.fp64_err == abs(s.price - b.price), q["kind"]
+
+
+def price_alone_kw(engine, q, **extra):
+    a, kw, s = q["args"], q["kw"], q["seed"]
+    if q["kind"] == "eur":
+        return engine.mc_eur(*a, seed=s, **extra)
+    if q["kind"] == "asia":
+        return engine.mc_asia(*a[:6], kw["M"], a[6], seed=s, **extra)
+    if q["kind"] == "amer":
+        return engine.mc_amer(*a[:6], kw["M"], a[6], seed=s, **extra)
+    return engine.mc_eur_multi(*a, kw["assets"], kw["rho"], seed=s, **extra)
+
+
 def test_binom_served(engine):
     with Server(engine, max_inflight=2) as srv:
         say("binom served")
@@ -219,21 +252,74 @@ def test_binom_served(engine):
     assert abs(r.price - want) / want < 1e-12
 
 
-def test_kept_path_stores_bounded_by_the_budget(engine):
-    """Slots keep their American path stores between requests only while the
-    stores kept by all slots fit the budget; past it a slot frees its store.
-    Results do not depend on which happened: each equals its lone call bit
-    for bit, in two rounds of the same requests."""
-    # An FP64 2^16 x 50 store is 2^16 * 50 * 8 B = 26 MB: with 64 MB and 4 slots
-    # at most two stay kept, so later requests free theirs.
-    N, M = 1 << 16, 50
-    seeds = [11, 12, 13, 14, 15, 16, 17, 18]
+def test_held_device_memory_never_exceeds_the_budget(engine):
+    """Kept path stores and in-flight requests share one budget: with FP64
+    2^22 x 50 American stores (1.7 GB each) and a 4 GiB budget on 4 slots, at
+    most two stores exist at a time, kept ones are freed to admit the next, the
+    charged bytes never pass the budget, no unit holds more than it was
+    charged, and every result equals its lone call bit for bit."""
+    N, M = 1 << 22, 50
+    seeds = [21, 22, 23, 24, 25]
     want = {s: engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision="fp64").price
             for s in seeds}
-    say("kept path stores, 64 MB budget, 4 slots")
-    with Server(engine, mem_budget_bytes=64 << 20, max_inflight=4) as srv:
-        for _ in range(2):
+    small = [engine.mc_eur(100, 100, 0.05, 0.2, 1.0, 1 << 20, "call", seed=s).price for s in seeds]
+    budget = 4 << 30
+    say("held memory, 4 GiB budget, 4 slots, FP64 2^22 stores")
+    with Server(engine, mem_budget_bytes=budget, max_inflight=4) as srv:
+        for rnd in range(2):
             ts = {s: srv.submit("amer", 100, 100, 0.06, 0.2, 1.0, N, "put", M=M, seed=s, precision="fp64")
                   for s in seeds}
+            te = [srv.submit("eur", 100, 100, 0.05, 0.2, 1.0, 1 << 20, "call", seed=s) for s in seeds]
             for s, t in ts.items():
-                assert srv.wait(t).price == want[s], s
+                assert srv.wait(t).price == want[s], (rnd, s)
+            assert [srv.wait(t).price for t in te] == small
+        st = srv.stats()
+    say("server stats", st)
+    assert st["budget_bytes"] == budget
+    assert st["peak_charged_bytes"] <= budget
+    assert st["peak_held_bytes"] <= budget
+    assert st["underestimates"] == 0
+    assert st["completed"] == 4 * len(seeds)
+    # two stores fit, three do not: kept stores were freed while the server ran
+    assert st["evictions"] >= 1
+    assert st["held_bytes"] <= budget
+
+
+def test_binomial_tables_are_charged(engine):
+    """Binomial requests hold resident tables (log table and chain prefix, or
+    the double-double log k! table): they are charged, so a budget just above
+    one request's tables serves them one at a time and frees what is kept."""
+    N = 1 << 20
+    want_chain = engine.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=True).price
+    want_dd = engine.binom_embar(100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=False).price
+    budget = 2 * (N + 1) * 8 + (4 << 20)
+    say("binomial under a tight budget")
+    with Server(engine, mem_budget_bytes=budget, max_inflight=3) as srv:
+        ts = [srv.submit("binom", 100, 100, 0.05, 0.2, 1.0, N, "call", ref_exact=(i % 2 == 0)) for i in range(6)]
+        got = [srv.wait(t).price for t in ts]
+        st = srv.stats()
+    assert got == [want_chain if i % 2 == 0 else want_dd for i in range(6)]
+    assert st["peak_charged_bytes"] <= budget and st["underestimates"] == 0
+    assert st["evictions"] >= 1
+
+
+@pytest.mark.parametrize("precision", ["fp32k", "fp64"])
+def test_full_size_range_served_mix_identical_alone(engine, precision):
+    """The served mix over the whole request range, 2^14..2^22 paths, in both
+    precisions, with a budget small enough that FP64 2^22 path stores are
+    freed and re-made: every result equals its lone call bit for bit."""
+    reqs = mixed_requests(13, 40, 14, 22)
+    for q in reqs:
+        q["kw"] = dict(q["kw"], precision=precision)
+    say("full-range mix alone", precision, [q["args"][5] for q in reqs])
+    alone = [price_alone(engine, q, precision=precision) for q in reqs]
+    budget = 3 << 30
+    with Server(engine, mem_budget_bytes=budget, max_inflight=6) as srv:
+        say("full-range mix served", precision)
+        tickets = [srv.submit(q["kind"], *q["args"], seed=q["seed"], **q["kw"]) for q in reqs]
+        served = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    for q, a, b in zip(reqs, alone, served):
+        assert a.price == b.price and a.std_err == b.std_err, (q["kind"], q["args"][5], a.price, b.price)
+    assert max(q["args"][5] for q in reqs) > (1 << 21)
+    assert st["peak_charged_bytes"] <= budget and st["underestimates"] == 0
