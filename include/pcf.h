@@ -172,6 +172,24 @@ int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
                 int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
                 pcf_result* res);
 
+/* pcf_mc_asia without device allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- tight serving loops and
+ * harnesses that forbid hipMalloc.  It runs pcf_mc_asia's kernels with the
+ * same launch shape, so its result equals pcf_mc_asia's bit for bit, but every
+ * device buffer those kernels use (job record, per-block partials, arrival
+ * counters, results) lives in one file-scope device array of 2 MiB + 5 KiB
+ * per GPU, held until the process ends.  Limits: Philox normals only (host
+ * normals would need a device buffer), no fp64_check, and at most 32768 grid
+ * blocks with the geometric control (65536 without); larger launches are
+ * refused with PCF_ERR_INVALID_ARG.  The context's stream, events and pinned
+ * staging (allocated by pcf_ctx_create) are used; the call allocates nothing.
+ * Calls on one GPU share the array and are serialised by a per-GPU lock (each
+ * call returns after its work completes); the call cannot be captured into a
+ * graph, as it synchronises its stream. */
+int pcf_mc_asia_noalloc(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                        int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
+                        pcf_result* res);
+
 /* American option by Longstaff-Schwartz on an antithetic path store.
  * Replaces double mc_amer(S0,E,r,sigma,T,N,M,payoff_fun), src/mc_amer.cpp:5-114
  * (paths: include/common.h:168-208).  N must be even.

@@ -190,6 +190,8 @@ SIGNATURES = {
     "pcf_binom_vanilla": (C.c_int, [_ctx, _d, _d, _d, _d, _d, _i64, _d, C.c_int32, C.POINTER(PcfOpts),
                                     C.POINTER(PcfResult)]),
     "pcf_mc_asia": (C.c_int, [_ctx, _d, _d, _d, _d, _d, _i64, _i32, _d, C.POINTER(PcfOpts), C.POINTER(PcfResult)]),
+    "pcf_mc_asia_noalloc": (C.c_int, [_ctx, _d, _d, _d, _d, _d, _i64, _i32, _d, C.POINTER(PcfOpts),
+                                      C.POINTER(PcfResult)]),
     "pcf_mc_amer": (C.c_int, [_ctx, _d, _d, _d, _d, _d, _i64, _i32, _d, C.POINTER(PcfOpts), C.POINTER(PcfResult)]),
     "pcf_mc_eur_multi": (C.c_int, [_ctx, _d, _d, _d, _d, _d, _i64, _d, _i32, _d, C.POINTER(PcfBasketExt),
                                    C.POINTER(PcfOpts), C.POINTER(PcfResult)]),

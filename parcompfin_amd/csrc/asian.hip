@@ -361,6 +361,120 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
     return PCF_OK;
 }
 
+// ---- the no-allocation entry point -----------------------------------------
+// pcf_mc_asia_noalloc runs the same kernels as pcf_mc_asia with every device
+// buffer they need -- the job record, the per-block compensated partials, the
+// arrival counters and the results -- in one file-scope device array, so a
+// call allocates no device memory.  Kernel arguments and host code use the
+// array's device address, obtained once per device with hipGetSymbolAddress.
+namespace {
+
+constexpr int kNoAllocMaxBlocks = 32768;  // grid blocks of one launch
+constexpr size_t kNoAllocPartialsBytes = (size_t)kNoAllocMaxBlocks * 4 * sizeof(pcf::CSum);  // 2 MiB
+constexpr size_t kNoAllocJobOff = kNoAllocPartialsBytes;
+constexpr size_t kNoAllocCounterOff = kNoAllocJobOff + kJobBytes;
+constexpr size_t kNoAllocOutOff =
+    kNoAllocCounterOff + (pcf::kCounterWords * sizeof(unsigned int) + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocOutOff + 256;
+
+__global__ void k_zero_words(unsigned int* __restrict__ p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+}  // namespace
+
+// The scratch array (2 MiB + 5 KiB per device, held until the process ends).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Per device: the array's device address, and a lock that serialises the
+// calls using it (each call returns only after its work is complete).
+struct ScratchSlot {
+    std::mutex mu;
+    char* ptr = nullptr;
+};
+ScratchSlot& scratch_slot(int device) {
+    static ScratchSlot slots[64];
+    return slots[device & 63];
+}
+
+}  // namespace
+
+extern "C" int pcf_mc_asia_noalloc(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
+                                   int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
+                                   pcf_result* res) {
+    if (!ctx || !res) return PCF_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    pcf_request q = pcf_make_request(PCF_KIND_ASIA, S0, E, r, sigma, T, N, payoff_fun, opts);
+    q.M = M;
+    *res = pcf_result{};
+    if (q.opts.rng != PCF_RNG_PHILOX)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "pcf_mc_asia_noalloc draws Philox normals (host normals would "
+                        "need a device buffer: use pcf_mc_asia)");
+    if (q.opts.fp64_check)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG, "pcf_mc_asia_noalloc does not make the FP64 comparison run");
+    if (ctx->device < 0 || ctx->device >= 64) return pcf_fail(ctx, PCF_ERR_NO_DEVICE, "device index");
+    AsiaPrep p;
+    int st = asia_prepare(ctx, q, p);
+    if (st) return st;
+    const int K = p.control ? 4 : 2;
+    if ((size_t)p.nblk * K * sizeof(pcf::CSum) > kNoAllocPartialsBytes)
+        return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
+                        "pcf_mc_asia_noalloc holds the partials of at most %zu blocks with %d sums (this launch "
+                        "has %d blocks of %d threads): use pcf_mc_asia",
+                        kNoAllocPartialsBytes / (K * sizeof(pcf::CSum)), K, p.nblk, p.B);
+    PCF_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    ScratchSlot& slot = scratch_slot(ctx->device);
+    std::lock_guard<std::mutex> sl(slot.mu);
+    if (!slot.ptr) {
+        void* dp = nullptr;
+        PCF_HIP_CHECK(ctx, hipGetSymbolAddress(&dp, HIP_SYMBOL(scratch)));
+        slot.ptr = static_cast<char*>(dp);
+    }
+    char* const base = slot.ptr;
+    unsigned int* const counter = reinterpret_cast<unsigned int*>(base + kNoAllocCounterOff);
+    AsiaJob* const djob = reinterpret_cast<AsiaJob*>(base + kNoAllocJobOff);
+    double* const dout = reinterpret_cast<double*>(base + kNoAllocOutOff);
+    AsiaJob* hj = static_cast<AsiaJob*>(ctx->h_jobs);
+    *hj = AsiaJob{};
+    hj->a = p.a;
+    hj->hz = nullptr;
+    hj->partials = reinterpret_cast<pcf::CSum*>(base);
+    hj->counter = counter;
+    hj->out = dout;
+    hj->first_block = 0;
+    hj->nblk = (unsigned int)p.nblk;
+    if ((st = pcf_timing_begin(ctx))) return st;
+    // the arrival counters start at zero (a fill kernel: no memset node)
+    k_zero_words<<<1, 256, 0, ctx->stream>>>(counter, pcf::kCounterWords);
+    PCF_HIP_CHECK(ctx, hipGetLastError());
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(djob, hj, sizeof(AsiaJob), hipMemcpyHostToDevice, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
+    const unsigned int grid = (unsigned int)p.nblk;
+    if (p.mode == kFp32k) {
+        if (p.control) k_asia<kFp32k, true><<<grid, p.B, 0, ctx->stream>>>(djob, 1);
+        else k_asia<kFp32k, false><<<grid, p.B, 0, ctx->stream>>>(djob, 1);
+    } else {
+        if (p.control) k_asia<kFp64, true><<<grid, p.B, 0, ctx->stream>>>(djob, 1);
+        else k_asia<kFp64, false><<<grid, p.B, 0, ctx->stream>>>(djob, 1);
+    }
+    PCF_HIP_CHECK(ctx, hipGetLastError());
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
+    PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_out, dout, kOutPerJob * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    pcf_result timing{};
+    float kms = 0.f, tms = 0.f;
+    PCF_HIP_CHECK(ctx, hipEventElapsedTime(&kms, ctx->ev_kbeg, ctx->ev_kend));
+    PCF_HIP_CHECK(ctx, hipEventElapsedTime(&tms, ctx->ev_begin, ctx->ev_end));
+    timing.kernel_ms = kms;
+    timing.total_ms = tms;
+    asia_result(p, ctx->h_out, N, M, p.B, 1, timing, *res);
+    return PCF_OK;
+}
+
 extern "C" int pcf_mc_asia(pcf_ctx* ctx, double S0, double E, double r, double sigma, double T,
                            int64_t N, int32_t M, double payoff_fun, const pcf_opts* opts,
                            pcf_result* res) {
