@@ -38,7 +38,11 @@ struct DevBuf {
         }
         bytes = want;
         if (zero) {
+            // hipMemset on device memory is asynchronous to the host and runs
+            // on the null stream, which the library's non-blocking streams do
+            // not wait for: complete it before the buffer is handed out.
             e = hipMemset(ptr, 0, want);
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;

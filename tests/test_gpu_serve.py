@@ -156,6 +156,40 @@ def test_american_requests_share_persistent_launches(engine):
     assert max(sizes) >= 4
 
 
+def test_concurrent_persistent_launches_match_lone_calls(engine):
+    """Persistent LSM launches from several host threads (one context each),
+    of 64-, 128- and 256-block grids, hand over to the device's persistent
+    stream in whatever order the threads reach it, each after its own inputs;
+    every result equals its lone call bit for bit, twice over."""
+    from parcompfin_amd import Engine
+    cases = []
+    for s in range(6):
+        cases.append((1 << 14, 50, 300 + s, "fp32k"))   # 64 blocks
+        cases.append((1 << 16, 50, 400 + s, "fp64"))    # 128 blocks
+    cases += [(1 << 20, 50, 500, "fp32k"), (1 << 18, 30, 501, "fp64"), (3 << 15, 20, 502, "fp32k")]
+    want = [engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision=p) for N, M, s, p in cases]
+    got = [None] * len(cases)
+    engines = [Engine(0) for _ in range(4)]
+
+    def work(k):
+        for i in range(k, len(cases), 4):
+            N, M, s, p = cases[i]
+            got[i] = engines[k].mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision=p)
+
+    say("concurrent persistent launches, 4 threads")
+    for _ in range(2):
+        threads = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=120)
+        for c, a, b in zip(cases, want, got):
+            assert b is not None and a.price == b.price and a.std_err == b.std_err, (c, a.price, b and b.price)
+            assert a.control_price == b.control_price and a.grid_blocks == b.grid_blocks
+    for e in engines:
+        e.close()
+
+
 def test_served_american_basis_requests(engine):
     """K-term regressions are served like any American request: same result as
     the direct call, alongside quadratic ones."""
