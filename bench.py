@@ -241,6 +241,20 @@ def traffic_of(kernel):
     return t["bytes_per_launch"] if t else None
 
 
+def counters_of(kernel):
+    """VALU/MFMA counter figures of a kernel from its committed rocprofv3 --pmc
+    summary (profiles/pmc_valu.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_valu.json")) as f:
+            c = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if c:
+        c = dict(c, source="profiles/k_asia_2p22x252_pmc_valu.md" if kernel.startswith("k_asia")
+                 else "profiles/k_basket_2p24_paths_pmc_valu.md")
+    return c
+
+
 def asia_roofline(eng, K=10):
     """k_asia<fp32k> at 2^22 x 252: algorithmic FP32 flops / HIP-event time."""
     N, M = 1 << 22, 252
@@ -254,7 +268,8 @@ def asia_roofline(eng, K=10):
             "kernel": "k_asia<fp32k> (mc_asia, 2^22 paths x 252 steps)", "achieved": ach,
             "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": traffic_of("k_asia<0>"),
             "kernel_ms": kms, "flops_per_launch": fl,
-            "flops_per_unit": "14 per path-step (normal 6, I update 4, S update 4)"}
+            "flops_per_unit": "14 per path-step (normal 6, I update 4, S update 4)",
+            "counters": counters_of("k_asia<0, false>")}
 
 
 def lsm_roofline(eng, K=10):
@@ -287,12 +302,14 @@ def basket_roofline(eng, K=10):
     kms = sum(r.kernel_ms for r in rs) / K
     fl = rs[0].flops
     ach = fl / (kms * 1e-3) / 1e12
-    return {"bound": "valu", "pipe": "Z L^T on MFMA (v_mfma_f32_32x32x2f32); issue-bound on the VALU by the "
-                                     "Philox normals and the per-asset exp",
+    return {"bound": "valu", "pipe": "exponents c0 + (c1 L) Z^T on MFMA (v_mfma_f32_32x32x2f32); the SIMD "
+                                     "cycles split between VALU issue (Philox normals, per-asset exp) and the "
+                                     "MFMA pipe, whose sum bounds the kernel",
             "kernel": "k_basket_mfma (mc_eur_multi d=32, 2^24 paths)", "achieved": ach,
             "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
             "traffic": traffic_of("k_basket_mfma"), "kernel_ms": kms, "flops_per_launch": fl,
-            "flops_per_unit": "d^2 + 12 d + 3 per path (normals 6d, Z L^T d(d+1), asset 5d, payoff 3)"}
+            "flops_per_unit": "d^2 + 12 d + 3 per path (normals 6d, Z L^T d(d+1), asset 5d, payoff 3)",
+            "counters": counters_of("k_basket_mfma")}
 
 
 def profile_one(eng, which, K, W):

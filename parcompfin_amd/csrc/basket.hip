@@ -10,11 +10,13 @@
 //    registers, B = L z with scalar-broadcast L, per-asset exp and the weighted
 //    sum fused; FP32/Kahan, FP64 and host-normal verification forms.
 //  * k_basket_mfma: FP32, 16 <= d <= 32.  One wavefront prices 32 paths per
-//    tile: B^T (32 assets x 32 paths) = L (32 x 32, zero padded) * Z^T on
-//    v_mfma_f32_32x32x2f32, 16 issues per tile.  Lane l feeds path l%32 and
-//    generates exactly the normals its half (l/32) contributes, so Z and B
-//    never touch memory; the epilogue (exp, weights, basket sum) runs on the
-//    16 accumulator registers, halves combined with one lane exchange.
+//    tile: the exponents X^T (32 assets x 32 paths) = c0 + (c1 L) Z^T, with
+//    L zero padded to 32 x 32 and each row scaled by c1 log2 e, on
+//    v_mfma_f32_32x32x2f32, 16 issues per tile, the accumulation starting
+//    from c0 log2 e.  Lane l feeds path l%32 and generates exactly the
+//    normals its half (l/32) contributes, so Z and X never touch memory; the
+//    epilogue (exp2, weights, basket sum) runs on the 16 accumulator
+//    registers, halves combined with one lane exchange.
 #include <cmath>
 #include <atomic>
 #include <cstring>
@@ -44,8 +46,9 @@ struct BasketArgs {
 // Device constants, one buffer: FP64 block then FP32 block.
 //   FP64: L[d*d] (row-major), c0[d] = (r - s^2/2)T, c1[d] = s sqrt(T) (or s),
 //         ws[d] = w S0
-//   FP32: L32[32*32] zero padded, c0f[32], c1f[32] (x log2 e), wsf[32]; for
-//         the FMA kernel the first d*d of Lf hold the d x d factor.
+//   FP32: Lf[d*d] (the FMA kernel's factor), L32[32*32] zero padded with row
+//         i scaled by c1[i] log2 e (the MFMA kernel's), c0f[32], c1f[32]
+//         (x log2 e), wsf[32].
 struct BasketLayout {
     size_t off_L64, off_c064, off_c164, off_ws64, off_Lf, off_Lpad, off_c0f, off_c1f, off_wsf, bytes;
     static BasketLayout make(int d) {
@@ -169,7 +172,6 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const float* __restrict__ Lpad,
                                                  const float* __restrict__ c0,
-                                                 const float* __restrict__ c1,
                                                  const float* __restrict__ ws,
                                                  pcf::CSum* __restrict__ partials,
                                                  unsigned int* __restrict__ counter,
@@ -179,17 +181,18 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
     const int j = lane & 31;  // path within the tile; also the A-operand row (asset)
     const int h = lane >> 5;  // k half fed by this lane
     // A operand: instruction s uses k = 8*(s/4) + 4*h + (s%4), i.e. element s%4 of
-    // Philox block 2*(s/4) + h; this lane supplies L[j][k].
+    // Philox block 2*(s/4) + h; this lane supplies the scaled factor's [j][k].
     float Areg[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) Areg[s] = Lpad[j * 32 + 8 * (s >> 2) + 4 * h + (s & 3)];
-    // Accumulator register r of this lane holds row i = 8*(r/4) + 4*h + (r%4).
-    float e0[16], e1[16], ew[16];
+    // Accumulator register r of this lane holds row i = 8*(r/4) + 4*h + (r%4);
+    // every tile's accumulation starts from base[r] = c0[i] log2 e.
+    v16f base;
+    float ew[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-        e0[r] = c0[i];
-        e1[r] = c1[i];
+        base[r] = c0[i];
         ew[r] = ws[i];
     }
     const int kblk = (a.d + 7) >> 3;  // k blocks of 8 actually used
@@ -199,12 +202,10 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
     const int64_t nwaves = ((int64_t)nblk * blockDim.x) >> 6;
     for (int64_t t = wave; t < ntiles; t += nwaves) {
         const int64_t p = (t << 5) + j;
-        v16f acc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        v16f acc = base;
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
-            if (cc < kblk) {
+            if (cc < 2 || cc < kblk) {  // d >= 16: blocks 0 and 1 always run
                 float z[4];
                 pcf::normals4<float>(a.seed, PCF_STREAM_BASKET, (uint64_t)p, (uint32_t)(2 * cc + h), z);
 #pragma unroll
@@ -214,7 +215,7 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
         }
         float part = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part = fmaf(ew[r], pcf::exp2_f32(fmaf(e1[r], acc[r], e0[r])), part);
+        for (int r = 0; r < 16; ++r) part = fmaf(ew[r], pcf::exp2_f32(acc[r]), part);
         const float basket = part + __shfl_xor(part, 32, 64);
         const float v = pcf::payoff<float>(basket, a.Ef, a.pff);
         if (h == 0 && p < a.N) {
@@ -230,11 +231,14 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
         pcf::write_moments<1>(tot, a.N, out);
 }
 
-__global__ __launch_bounds__(kBasketThreads) void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
+// Four waves per SIMD: the compiler then fits the kernel in 106 VGPRs and
+// no AGPRs without spills (left to itself it took 102 VGPRs + 32 AGPRs, 3 waves).
+__global__ __launch_bounds__(kBasketThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
     __shared__ pcf::CSum lds[(kBasketThreads / pcf::kWave) * 2];
     const int j = pcf::find_job(jobs, njobs, blockIdx.x);
     const BasketJob& J = jobs[j];
-    basket_mfma_body(J.a, (const float*)J.Lm, (const float*)J.c0, (const float*)J.c1, (const float*)J.ws,
+    basket_mfma_body(J.a, (const float*)J.Lm, (const float*)J.c0, (const float*)J.ws,
                      J.partials, J.counter, J.out, blockIdx.x - J.first_block, J.nblk, lds);
 }
 
@@ -427,9 +431,6 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
             L64[i] = L[i];
             Lf[i] = (float)L[i];
         }
-        if (d <= 32)
-            for (int i = 0; i < d; ++i)
-                for (int k = 0; k < d; ++k) Lpad[i * 32 + k] = (float)L[i * d + k];
         for (int i = 0; i < d; ++i) {
             const double sg = (ext && ext->sigma) ? ext->sigma[i] : q.sigma;
             const double s0 = (ext && ext->S0) ? ext->S0[i] : q.S0;
@@ -443,6 +444,10 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
             c0f[i] = (float)(c064[i] * log2e);
             c1f[i] = (float)(c164[i] * log2e);
             wsf[i] = (float)ws64[i];
+            // The MFMA kernel's factor carries each row's c1 log2 e, so its
+            // accumulator holds the exponent itself (rounded once from FP64).
+            if (d <= 32)
+                for (int k = 0; k < d; ++k) Lpad[i * 32 + k] = (float)(L[i * d + k] * (c164[i] * log2e));
         }
         std::lock_guard<std::mutex> lk(consts_mu());
         if (consts_cache().size() > 1024) consts_cache().clear();

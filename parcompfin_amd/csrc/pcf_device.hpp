@@ -19,6 +19,13 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c as one gfx950 v_bitop3_b32 (truth table 0x96, the parity of the
+// three input bits).  Written as two ^ the compiler issues two v_xor_b32,
+// and the Monte Carlo kernels are bound by VALU issue, a third of it Philox.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Each round needs both halves of two 32x32 products; written as one 64-bit
 // product so gfx950 issues a single v_mad_u64_u32 per product (measured
 // ~2.4x the throughput of a v_mul_lo_u32 + v_mul_hi_u32 pair).
@@ -28,9 +35,16 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
         U4 n;
-        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        if (round == 0) {
+            // Counter words are often wave-uniform (block, stream): left as
+            // plain xors, the compiler keeps this round on the scalar unit.
+            n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+            n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        } else {
+            n.x = xor3((uint32_t)(p1 >> 32), c.y, k0);
+            n.z = xor3((uint32_t)(p0 >> 32), c.w, k1);
+        }
         n.y = (uint32_t)p1;
-        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
         n.w = (uint32_t)p0;
         c = n;
         k0 += 0x9E3779B9u;
