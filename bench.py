@@ -8,11 +8,14 @@ single-GPU configurations -- European call (S0=K=100, r=0.05, sigma=0.2),
 basket call on 8 equicorrelated assets (rho=0.3, sigma_i ~ U[0.15,0.35]),
 arithmetic Asian call (sigma=0.25, M=252) and American put by Longstaff-
 Schwartz (r=0.06, sigma=0.2, M=50 exercise dates, path store on the device)
--- with path counts log-uniform in [2^14, 2^22].  One step submits the whole
-set to the continuous-batching server (pcf_server_*) and waits for every
-result; value = sum over requests of paths x steps (1 for single-step
+-- with path counts log-uniform in [2^14, 2^22].  One step is the whole set
+served by the continuous-batching server (pcf_server_*); value = sum over
+requests of paths x steps (1 for single-step
 products, M otherwise) / step time, in the production FP32/Kahan form.  The
-same steps in the plain FP64 form give the FP32-vs-FP64 ratio.  A second
+K timed steps run under sustained load: their K sets are submitted back to
+back and every result is waited for, as a server sees a steady stream; the
+closed-loop time (one set at a time, waiting for each) is reported beside
+it.  The same steps in the plain FP64 form give the FP32-vs-FP64 ratio.  A second
 phase feeds a Poisson stream at ~75% of the measured capacity and reports
 p50/p99 latency overall and for 2^16-path requests.
 
@@ -30,12 +33,15 @@ same kernel, each scaled by what the counter reads for a copy of known size
 roofline object is also listed under "rooflines".
 
 cpu_baseline: the reference's own OpenMP programs (src/mc_eur_omp.cpp,
-src/mc_asia_omp.cpp, src/mc_amer_omp.cpp built from source at -O2 -fopenmp
-into oracle/_ref) on a bounded sample of the same mix (the reference basket
-program needs Eigen/Boost, absent here, so basket requests are left out of
-the sample); paths x steps / total T_calculation, also per kind ("by_kind",
-repeated in the per-configuration objects as cpu_reference_*), and one run
-each of the reference's binomial programs ("binomial").
+src/mc_asia_omp.cpp, src/mc_amer_omp.cpp, src/binom_embar_omp.cpp built from
+source at -O2 -fopenmp into oracle/_ref; the serial lattice programs) on this
+host's cores (at most 16, the thread count and CPU model stated), each at its
+configuration's own size, best of 3 T_calculation (binom_embar at N = 10^6
+once, with --cpu-full).  value = the served mix's European/Asian/American
+requests (paths x steps) over the time those per-kind rates give; the
+reference basket program needs Eigen/Boost, absent here, so basket requests
+are left out.  Each leg is also attached to its configuration's object in
+per_config with the GPU/CPU ratio at equal size.
 """
 import argparse
 import json
@@ -102,10 +108,22 @@ def path_steps(mix):
 
 
 def serve_once(srv, mix, precision):
+    """One closed-loop step: submit the set, wait for every result."""
     t0 = time.perf_counter()
     tickets = [srv.submit(q["kind"], *q["args"], seed=q["seed"], precision=precision, **q["kw"]) for q in mix]
     res = [srv.wait(t) for t in tickets]
     return time.perf_counter() - t0, res
+
+
+def serve_steps(srv, mix, precision, K):
+    """K steps under sustained load: the K sets are submitted back to back
+    (the server keeps working while later sets arrive), then every result is
+    waited for.  Returns the elapsed time and the last set's results."""
+    t0 = time.perf_counter()
+    tickets = [[srv.submit(q["kind"], *q["args"], seed=q["seed"], precision=precision, **q["kw"]) for q in mix]
+               for _ in range(K)]
+    res = [[srv.wait(t) for t in ts] for ts in tickets]
+    return time.perf_counter() - t0, res[-1]
 
 
 def poisson_phase(srv, mix, rate, precision="fp32k"):
@@ -524,16 +542,21 @@ def main():
     mix = make_mix(2026, args.requests)
     ps = path_steps(mix)
     srv = Server(eng, mem_budget_bytes=16 << 30, max_inflight=args.slots)
-    results = {}
+    results, sstats, closed = {}, {}, {}
     for prec in ("fp32k", "fp64"):
         for _ in range(W):
             serve_once(srv, mix, prec)
-        ts, rs = [], None
-        for _ in range(K):
-            t, rs = serve_once(srv, mix, prec)
-            ts.append(t)
-        results[prec] = (ts, rs)
-        log(f"mix {prec}: {1e3 * sum(ts) / K:.2f} ms/step, {ps * K / sum(ts):.4g} paths·steps/s")
+        # closed loop: one set at a time (reported beside the sustained figure)
+        closed[prec] = 1e3 * sum(serve_once(srv, mix, prec)[0] for _ in range(K)) / K
+        # sustained load: the timed K steps
+        t, rs = serve_steps(srv, mix, prec, K)
+        results[prec] = ([t], rs)
+        # cumulative server counters after this precision's steps
+        sstats[prec] = {k: v for k, v in srv.stats().items()
+                        if k in ("evictions", "underestimates", "peak_held_bytes", "peak_charged_bytes",
+                                 "budget_bytes", "completed")}
+        log(f"mix {prec}: {1e3 * t / K:.2f} ms/step sustained ({closed[prec]:.2f} closed loop), "
+            f"{ps * K / t:.4g} paths·steps/s")
     t32, r32 = results["fp32k"]
     t64, r64 = results["fp64"]
     value = ps * K / sum(t32)
@@ -602,10 +625,12 @@ def main():
         "rooflines": rooflines,
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": 1e3 * sum(t64) / K, "fp32k_over_fp64": value / value64},
+        "closed_loop_ms_per_step": closed,
         "accuracy": {"max_abs_fp32k_minus_fp64_in_se": max(dse) if dse else None,
                      "mean_abs_fp32k_minus_fp64_in_se": sum(dse) / len(dse) if dse else None},
         "latency": latency,
         "mix_by_kind": by_kind,
+        "server_stats": sstats,
         "per_config": configs,
         "autotune": tuned,
         "device": info["name"],

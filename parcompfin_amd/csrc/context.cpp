@@ -4,6 +4,8 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <thread>
+
 #include "pcf_internal.hpp"
 
 int pcf_fail(pcf_ctx* ctx, int code, const char* fmt, ...) {
@@ -202,12 +204,22 @@ int pcf_timing_begin(pcf_ctx* ctx) {
     return PCF_OK;
 }
 
+hipError_t pcf_wait_event(hipEvent_t ev) {
+    for (;;) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e != hipErrorNotReady) return e;
+        std::this_thread::yield();
+    }
+}
+
 int pcf_timing_end(pcf_ctx* ctx, int nout, pcf_result* res) {
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
     if (nout > 0)
         PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_out, ctx->out.ptr, (size_t)nout * sizeof(double),
                                           hipMemcpyDeviceToHost, ctx->stream));
-    PCF_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    // everything this call enqueued is done once ev_ready is
+    PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ready, ctx->stream));
+    PCF_HIP_CHECK(ctx, pcf_wait_event(ctx->ev_ready));
     float kms = 0.f, tms = 0.f;
     PCF_HIP_CHECK(ctx, hipEventElapsedTime(&kms, ctx->ev_kbeg, ctx->ev_kend));
     PCF_HIP_CHECK(ctx, hipEventElapsedTime(&tms, ctx->ev_begin, ctx->ev_end));

@@ -1485,16 +1485,18 @@ size_t lsm_dyn_lds_bytes(int threads) {
 }
 
 template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
-hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStream_t s, int device) {
+hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStream_t s, hipEvent_t ev0,
+                               hipEvent_t ev1) {
     LsmPersistArgs a = pa;
     void* args[] = {&a};
     // A plain launch: the grid fits the device's co-resident capacity (checked
     // by the caller), persistent launches run one after another on one stream
     // per device, and every other kernel of the library finishes on its own,
     // so all blocks become resident -- at worst after a co-running kernel
-    // ends.  Every wait is bounded all the same.
-    return hipLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, dim3((unsigned)grid),
-                           dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s);
+    // ends.  Every wait is bounded all the same.  ev0 / ev1 take the kernel's
+    // own start and end times (no marker packets between consecutive launches).
+    return hipExtLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, dim3((unsigned)grid),
+                              dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s, ev0, ev1, 0);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
@@ -1532,29 +1534,29 @@ bool lds_fits_t(int B, int device) {
 // ref == false, checked by the caller).
 template <int PMAX, int K>
 hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid, int B,
-                               hipStream_t s, int dev) {
-    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(pa, grid, B, s, dev);
+                               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(pa, grid, B, s, e0, e1);
     if constexpr (K == 3) {
         if (src == kHost)
-            return ref ? launch_persistent_t<double, kHost, true, PMAX>(pa, grid, B, s, dev)
-                       : launch_persistent_t<double, kHost, false, PMAX>(pa, grid, B, s, dev);
-        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(pa, grid, B, s, dev)
-                   : launch_persistent_t<double, kPhilox, false, PMAX>(pa, grid, B, s, dev);
+            return ref ? launch_persistent_t<double, kHost, true, PMAX>(pa, grid, B, s, e0, e1)
+                       : launch_persistent_t<double, kHost, false, PMAX>(pa, grid, B, s, e0, e1);
+        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(pa, grid, B, s, e0, e1)
+                   : launch_persistent_t<double, kPhilox, false, PMAX>(pa, grid, B, s, e0, e1);
     } else {
         (void)ref;
-        return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(pa, grid, B, s, dev)
-                            : launch_persistent_t<double, kPhilox, false, PMAX, K>(pa, grid, B, s, dev);
+        return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(pa, grid, B, s, e0, e1)
+                            : launch_persistent_t<double, kPhilox, false, PMAX, K>(pa, grid, B, s, e0, e1);
     }
 }
 template <int PMAX>
 hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid,
-                             int B, hipStream_t s, int dev) {
+                             int B, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     switch (K) {
-        case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, dev);
-        case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, dev);
-        case 5: return launch_persistent_k<PMAX, 5>(f64, src, ref, pa, grid, B, s, dev);
-        case 6: return launch_persistent_k<PMAX, 6>(f64, src, ref, pa, grid, B, s, dev);
-        default: return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, dev);
+        case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, e0, e1);
+        case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, e0, e1);
+        case 5: return launch_persistent_k<PMAX, 5>(f64, src, ref, pa, grid, B, s, e0, e1);
+        case 6: return launch_persistent_k<PMAX, 6>(f64, src, ref, pa, grid, B, s, e0, e1);
+        default: return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
     }
 }
 
@@ -1775,17 +1777,25 @@ hipError_t build_lsm_graph_k(bool f64, int src, const LsmBufs& b, int M, int gri
 
 }  // namespace
 
-int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact) {
+PcfAmerBytes pcf_amer_bytes(int64_t N, int M, int precision, int rng, int ref_exact) {
     const bool f64 = ref_exact || rng == PCF_RNG_HOST || precision == PCF_FP64;
     const int64_t rs = f64 ? 8 : 4;
     pcf_opts o;
     pcf_opts_init(&o);
     const LsmShape sh = lsm_shape(N, f64 ? 1 : 0, o);
     const int64_t grid_max = std::max(sh.grid_paths, sh.grid_step);
-    // + coefficients, discounts, parameters and sync words + the persistent
+    PcfAmerBytes b;
+    b.store = (int64_t)M * sh.cap * rs;
+    b.state = sh.cap * (4 + 2 * rs);
+    // coefficients, discounts, parameters and sync words + the persistent
     // form's partials: at most 4096 co-resident blocks x 448 B (6-term basis)
-    return (int64_t)M * sh.cap * rs + sh.cap * (4 + 2 * rs) + grid_max * 10 * 16 +
-           8 * (9 * (int64_t)(M + 1)) + 8192 + 4096 * 448;
+    b.rest = grid_max * 10 * 16 + 8 * (9 * (int64_t)(M + 1)) + 8192 + 4096 * 448;
+    return b;
+}
+
+int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact) {
+    const PcfAmerBytes b = pcf_amer_bytes(N, M, precision, rng, ref_exact);
+    return b.store + b.state + b.rest;
 }
 
 // Releases a context's captured LSM graphs (called from pcf_ctx_destroy).
@@ -1955,6 +1965,9 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
     std::vector<double> host_blk((CS + 1) * (size_t)(M + 1), 0.0);
     for (int k = 0; k <= M; ++k) host_blk[CS * (size_t)(M + 1) + k] = std::exp(-r * dt * k);
 
+    // Nothing left on the context's stream from earlier calls (checked before
+    // this call enqueues its own begin event).
+    const bool ctx_idle = hipStreamQuery(ctx->stream) == hipSuccess;
     if ((st = pcf_timing_begin(ctx))) return st;
     if (src == kHost && (st = pcf_upload_normals(ctx, &o, (N / 2) * (int64_t)M))) return st;
     b.hw = src == kHost ? ctx->normals.as<double>() : nullptr;
@@ -2005,23 +2018,31 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         for (int k = 0; k <= M && k < kDiscArgs; ++k) pa.disc[k] = host_blk[CS * (size_t)(M + 1) + k];
         hipStream_t cs = nullptr;
         PCF_HIP_CHECK(ctx, coop_stream(ctx->device, &cs));
-        // inputs ready on the context's stream -> the launch on the shared
-        // stream -> completion waited on the context's stream
-        PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ready, ctx->stream));
+        // Inputs enqueued on the context's stream (host normals, the
+        // coefficient block) are handed over with an event; with none, the
+        // launch needs no wait.  The host then waits for the kernel's end
+        // event itself: every later use of this context's buffers is enqueued
+        // after that wait.
+        const bool handover = !ctx_idle || src == kHost || M + 1 > kDiscArgs;
+        if (handover) PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ready, ctx->stream));
         {
             std::lock_guard<std::mutex> pl(persistent_mu());
-            PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
-            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, cs));
+            if (handover) PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
+            hipEvent_t e0 = ctx->ev_kbeg, e1 = ctx->ev_kend;
             hipError_t le;
-            if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
-            else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
-            else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
-            else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, ctx->device);
+            if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
-            PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, cs));
         }
-        PCF_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_kend, 0));
-        if ((st = pcf_timing_end(ctx, 0, res))) return st;
+        PCF_HIP_CHECK(ctx, pcf_wait_event(ctx->ev_kend));
+        PCF_HIP_CHECK(ctx, pcf_wait_event(ctx->ev_begin));
+        float kms = 0.f, tms = 0.f;
+        PCF_HIP_CHECK(ctx, hipEventElapsedTime(&kms, ctx->ev_kbeg, ctx->ev_kend));
+        PCF_HIP_CHECK(ctx, hipEventElapsedTime(&tms, ctx->ev_begin, ctx->ev_kend));
+        res->kernel_ms = kms;
+        res->total_ms = tms;
         res->grid_blocks = pgrid;
         res->block_size = PB;
     }

@@ -134,6 +134,12 @@ int pcf_check_opts(pcf_ctx* ctx, const pcf_opts* o);
 // Copies `count` FP64 normals to ctx->normals.
 int pcf_upload_normals(pcf_ctx* ctx, const pcf_opts* o, int64_t count);
 
+// Waits for an event by polling it from this thread.  A blocking wait sleeps
+// in the driver until the completion interrupt and wakes a few hundred
+// microseconds after the work ended (measured between back-to-back served
+// requests); polling hands the next request to the device without that delay.
+hipError_t pcf_wait_event(hipEvent_t ev);
+
 // Records the begin event, returns status.  Timing: begin/kbeg/kend/end.
 int pcf_timing_begin(pcf_ctx* ctx);
 // Copies `nout` doubles of results to ctx->h_out, waits, fills timing fields.
@@ -206,7 +212,13 @@ uint64_t pcf_asia_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
 int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
 uint64_t pcf_basket_batch_key(pcf_ctx* ctx, const pcf_request& q, int* status);
 int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result* res, int* status);
-// Device bytes an American request holds while in flight (path store + state).
+// Device bytes an American request holds while in flight: the path store and
+// per-path state (grow-only buffers a context keeps between requests) and the
+// rest (coefficients, partials, sync words); the footprint is their sum.
+struct PcfAmerBytes {
+    int64_t store = 0, state = 0, rest = 0;
+};
+PcfAmerBytes pcf_amer_bytes(int64_t N, int M, int precision, int rng, int ref_exact);
 int64_t pcf_amer_footprint(int64_t N, int M, int precision, int rng, int ref_exact);
 
 // opts.fp64_check on a PCF_FP32K Philox request of a Monte Carlo kind: prices

@@ -9,11 +9,13 @@
 // hold -- path stores and per-path state, reduction partials, binomial
 // tables -- and the charges of all slots never exceed mem_budget_bytes.  An
 // idle slot is charged what it keeps from its last request (measured); a
-// busy slot what it kept plus the footprint of the unit it runs (the full
-// path store for the American, the tables for the binomial, partials
-// otherwise), an upper bound of what the unit can add.  A free slot takes
+// busy slot what it kept plus an upper bound of what the unit can add: for
+// the American the growth of the kept path store and state it reuses (the
+// buffers only grow) plus its other buffers; the tables for the binomial;
+// partials otherwise.  A free slot takes
 // the oldest queued request whose charge fits (smaller requests may pass one
-// that does not fit yet), first by dropping its own kept buffers, then by
+// that does not fit yet; the oldest American request passes all others while
+// fewer than two slots hold one), first by dropping its own kept buffers, then by
 // freeing idle slots' kept buffers, least recently used first; it also takes
 // every further queued request with the same batch key (single-launch kinds
 // only: European, Asian, basket) up to kMaxBatch, and prices them in one
@@ -67,6 +69,8 @@ struct pcf_server {
     // its charge against the budget, busy flag and last use (for eviction).
     std::vector<int64_t> held, charge;
     std::vector<char> busy;
+    std::vector<char> busy_amer;  // the slot's unit is an American request
+    std::vector<int64_t> kept_store, kept_state;  // American path store / state bytes among held
     std::vector<uint64_t> last_used;
     uint64_t use_clock = 0;
     int64_t charged = 0;  // sum of charge
@@ -83,6 +87,9 @@ struct pcf_server {
 };
 
 namespace {
+
+// American units a free slot takes ahead of older requests (see take_unit).
+constexpr int kAmerAhead = 2;
 
 int64_t footprint_of(const pcf_request& q, int64_t nblk_hint) {
     switch (q.kind) {
@@ -191,12 +198,44 @@ bool take_unit(pcf_server* srv, int slot, std::vector<Item>& unit, bool* drop_ow
             idle.push_back(s);
             evictable += srv->held[s];
         }
-    for (size_t i = 0; i < q.size(); ++i) {
-        const int64_t fp = q[i].footprint;
+    // Candidates in queue order, except that the oldest American request comes
+    // first while fewer than kAmerAhead slots hold one.  American requests run
+    // one after another on the device's persistent stream (their grids need
+    // every CU); keeping the next one enqueued behind the running one keeps
+    // that chain, the longest in a mixed load, from waiting on the host.
+    std::vector<size_t> order;
+    order.reserve(q.size());
+    int amer_busy = 0;
+    for (size_t s = 0; s < srv->slots.size(); ++s) amer_busy += srv->busy_amer[s] ? 1 : 0;
+    size_t first_amer = q.size();
+    if (amer_busy < kAmerAhead)
+        for (size_t i = 0; i < q.size(); ++i)
+            if (q[i].req.kind == PCF_KIND_AMER) {
+                first_amer = i;
+                order.push_back(i);
+                break;
+            }
+    for (size_t i = 0; i < q.size(); ++i)
+        if (i != first_amer) order.push_back(i);
+    for (size_t i : order) {
+        // An American request reuses the path store and state the slot keeps
+        // (its buffers only grow): kept, it adds only the growth beyond them.
+        const int64_t fp_drop = q[i].footprint;
+        int64_t fp_keep = fp_drop;
+        if (q[i].req.kind == PCF_KIND_AMER) {
+            const pcf_request& r = q[i].req;
+            const PcfAmerBytes b = pcf_amer_bytes(r.N, r.M, r.opts.precision, r.opts.rng, r.opts.ref_exact);
+            fp_keep = std::max<int64_t>(0, b.store - srv->kept_store[slot]) +
+                      std::max<int64_t>(0, b.state - srv->kept_state[slot]) + b.rest;
+        }
+        // Keep the slot's own buffers if that fits, after freeing idle slots'
+        // if need be; else drop them; else the request waits.
         bool drop = false;
-        if (others + own + fp > srv->budget) {
+        int64_t fp = fp_keep;
+        if (others + own + fp_keep > srv->budget && others - evictable + own + fp_keep > srv->budget) {
+            if (others - evictable + fp_drop > srv->budget) continue;
             drop = own > 0;
-            if (others + fp > srv->budget && others - evictable + fp > srv->budget) continue;
+            fp = fp_drop;
         }
         int64_t room = srv->budget - others - (drop ? 0 : own);
         if (fp > room) {
@@ -207,13 +246,14 @@ bool take_unit(pcf_server* srv, int slot, std::vector<Item>& unit, bool* drop_ow
                 release_request_buffers(srv->slots[s]);
                 room += srv->held[s];
                 srv->held[s] = 0;
+                srv->kept_store[s] = srv->kept_state[s] = 0;
                 set_charge(srv, s, 0);
                 ++srv->st.evictions;
             }
         }
         Item head = q[i];
         q.erase(q.begin() + (long)i);
-        int64_t total = head.footprint;
+        int64_t total = fp;
         unit.push_back(head);
         if (head.key != 0) {
             for (size_t j = i; j < q.size() && (int)unit.size() < kMaxBatch;) {
@@ -229,9 +269,11 @@ bool take_unit(pcf_server* srv, int slot, std::vector<Item>& unit, bool* drop_ow
         *drop_own = drop;
         if (drop) {
             srv->held[slot] = 0;
+            srv->kept_store[slot] = srv->kept_state[slot] = 0;
             ++srv->st.evictions;
         }
         srv->busy[slot] = 1;
+        srv->busy_amer[slot] = head.req.kind == PCF_KIND_AMER;
         set_charge(srv, slot, (drop ? 0 : own) + total);
         const double t = srv->now_ms();
         for (auto& it : unit) it.start_ms = t;
@@ -257,10 +299,12 @@ void worker_main(pcf_server* srv, int slot) {
         }
         if (drop_own) release_request_buffers(ctx);
         run_unit(srv, slot, ctx, unit, fin);
-        int64_t now_held;
+        int64_t now_held, now_store, now_state;
         {
             std::lock_guard<std::recursive_mutex> lk(ctx->mu);
             now_held = request_bytes(ctx);
+            now_store = (int64_t)ctx->lsm_store.bytes;
+            now_state = (int64_t)ctx->lsm_state.bytes;
         }
         bool over = false;
         {
@@ -276,13 +320,16 @@ void worker_main(pcf_server* srv, int slot) {
         }
         if (over) {
             release_request_buffers(ctx);
-            now_held = 0;
+            now_held = now_store = now_state = 0;
         }
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             srv->held[slot] = now_held;
+            srv->kept_store[slot] = now_store;
+            srv->kept_state[slot] = now_state;
             set_charge(srv, slot, now_held);
             srv->busy[slot] = 0;
+            srv->busy_amer[slot] = 0;
             srv->last_used[slot] = ++srv->use_clock;
             int64_t sum = 0;
             for (int64_t h : srv->held) sum += h;
@@ -310,6 +357,9 @@ extern "C" int pcf_server_create(pcf_ctx* ctx, const pcf_server_config* cfg, pcf
     srv->held.assign(nslots, 0);
     srv->charge.assign(nslots, 0);
     srv->busy.assign(nslots, 0);
+    srv->busy_amer.assign(nslots, 0);
+    srv->kept_store.assign(nslots, 0);
+    srv->kept_state.assign(nslots, 0);
     srv->last_used.assign(nslots, 0);
     srv->st.budget_bytes = srv->budget;
     srv->st.slots = nslots;

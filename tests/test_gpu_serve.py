@@ -289,13 +289,15 @@ def test_binom_served(engine):
 def test_held_device_memory_never_exceeds_the_budget(engine):
     """Kept path stores and in-flight requests share one budget: with FP64
     2^22 x 50 American stores (1.7 GB each) and a 4 GiB budget on 4 slots, at
-    most two stores exist at a time, kept ones are freed to admit the next, the
-    charged bytes never pass the budget, no unit holds more than it was
-    charged, and every result equals its lone call bit for bit."""
+    most two stores exist at a time; a slot reuses the store it keeps; a
+    request with a 3.4 GB store (M = 100) then needs the others' kept stores
+    freed.  The charged bytes never pass the budget, no unit holds more than it
+    was charged, and every result equals its lone call bit for bit."""
     N, M = 1 << 22, 50
     seeds = [21, 22, 23, 24, 25]
     want = {s: engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision="fp64").price
             for s in seeds}
+    want_big = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, 100, "put", seed=26, precision="fp64").price
     small = [engine.mc_eur(100, 100, 0.05, 0.2, 1.0, 1 << 20, "call", seed=s).price for s in seeds]
     budget = 4 << 30
     say("held memory, 4 GiB budget, 4 slots, FP64 2^22 stores")
@@ -307,14 +309,17 @@ def test_held_device_memory_never_exceeds_the_budget(engine):
             for s, t in ts.items():
                 assert srv.wait(t).price == want[s], (rnd, s)
             assert [srv.wait(t).price for t in te] == small
+        tb = srv.submit("amer", 100, 100, 0.06, 0.2, 1.0, N, "put", M=100, seed=26, precision="fp64")
+        assert srv.wait(tb).price == want_big
         st = srv.stats()
     say("server stats", st)
     assert st["budget_bytes"] == budget
     assert st["peak_charged_bytes"] <= budget
     assert st["peak_held_bytes"] <= budget
     assert st["underestimates"] == 0
-    assert st["completed"] == 4 * len(seeds)
-    # two stores fit, three do not: kept stores were freed while the server ran
+    assert st["completed"] == 4 * len(seeds) + 1
+    # the M = 100 store and a kept 2^22 x 50 store do not fit beside a second
+    # kept one: kept stores were freed while the server ran
     assert st["evictions"] >= 1
     assert st["held_bytes"] <= budget
 
