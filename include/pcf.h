@@ -311,9 +311,11 @@ const char* pcf_server_last_error(pcf_server* srv);
 /* Device memory of a server.  Each slot is charged for the request-sized
  * buffers it holds or may hold (path stores and per-path state, reduction
  * partials, binomial tables): what it kept from its last request when idle,
- * plus the footprint of the unit it runs when busy.  The sum of the charges
- * never exceeds mem_budget_bytes: kept buffers are freed (the slot's own,
- * then idle slots' least recently used) before a request waits. */
+ * plus, when busy, what the unit it runs can add (for an American request
+ * the growth of the kept path store and state it reuses, plus its other
+ * buffers).  The sum of the charges never exceeds mem_budget_bytes: idle
+ * slots' kept buffers (least recently used first), then the slot's own, are
+ * freed before a request waits. */
 typedef struct pcf_server_stats {
     int64_t budget_bytes;         /* mem_budget_bytes in force */
     int64_t charged_bytes;        /* sum of the slots' charges now */
