@@ -9,14 +9,14 @@
 //  * k_basket_fma<Real, DMAX, MODE, FULL>: one path per lane; d normals in
 //    registers, B = L z with scalar-broadcast L, per-asset exp and the weighted
 //    sum fused; FP32/Kahan, FP64 and host-normal verification forms.
-//  * k_basket_mfma: FP32, 16 <= d <= 32.  One wavefront prices 32 paths per
-//    tile: the exponents X^T (32 assets x 32 paths) = c0 + (c1 L) Z^T, with
+//  * k_basket_mfma: FP32, 16 <= d <= 32.  One wavefront prices 16 paths per
+//    tile: the exponents X^T (32 assets x 16 paths) = c0 + (c1 L) Z^T, with
 //    L zero padded to 32 x 32 and each row scaled by c1 log2 e, on
-//    v_mfma_f32_32x32x2f32, 16 issues per tile, the accumulation starting
-//    from c0 log2 e.  Lane l feeds path l%32 and generates exactly the
-//    normals its half (l/32) contributes, so Z and X never touch memory; the
-//    epilogue (exp2, weights, basket sum) runs on the 16 accumulator
-//    registers, halves combined with one lane exchange.
+//    v_mfma_f32_16x16x4f32 in two blocks of 16 assets, skipping the factor's
+//    zero upper-right block, the accumulation starting from c0 log2 e.  Each
+//    lane generates exactly the normals it feeds, so Z and X never touch
+//    memory; the epilogue (exp2, weights, basket sum) runs on the 8
+//    accumulator registers, the four lane groups combined by lane exchange.
 #include <cmath>
 #include <atomic>
 #include <cstring>
@@ -40,6 +40,7 @@ struct BasketArgs {
     float Ef, pff;
     int64_t N;
     int32_t d;
+    int32_t full;  // the factor is not lower triangular (eigen fallback)
     uint64_t seed;
 };
 
@@ -168,8 +169,18 @@ __global__ __launch_bounds__(kBasketThreads) void k_basket_fma(const BasketJob* 
                                             blockIdx.x - J.first_block, J.nblk, lds);
 }
 
-typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
+// One wavefront prices a tile of 16 paths with v_mfma_f32_16x16x4f32 (A: 16
+// rows x 4 k, lane l supplies row l%16, k l/16; B: 4 k x 16 columns, lane l
+// supplies k l/16, column l%16; D: lane l holds rows 4(l/16)..+3 of column
+// l%16).  Rows are assets in two blocks of 16 (a = 0, 1), columns the tile's
+// paths.  Step s (0..7) of a block sums over k = 16(s/4) + 4g + s%4 for the
+// four lane groups g = l/16: lane (g, j) feeds the 8 normals of Philox blocks
+// g and 4 + g of path j, whole blocks, in that order.  A Cholesky factor is
+// lower triangular, so asset block 0 needs only k < 16, steps 0..3: 12 MFMAs
+// of 32 cycles per tile instead of the 16 of 64 a padded 32 x 32 x 2 form
+// takes.  A full factor (the eigen fallback) runs block 0's steps 4..7 too.
 __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const float* __restrict__ Lpad,
                                                  const float* __restrict__ c0,
                                                  const float* __restrict__ ws,
@@ -178,47 +189,63 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
                                                  double* __restrict__ out, unsigned int blk,
                                                  unsigned int nblk, pcf::CSum* lds) {
     const int lane = threadIdx.x & 63;
-    const int j = lane & 31;  // path within the tile; also the A-operand row (asset)
-    const int h = lane >> 5;  // k half fed by this lane
-    // A operand: instruction s uses k = 8*(s/4) + 4*h + (s%4), i.e. element s%4 of
-    // Philox block 2*(s/4) + h; this lane supplies the scaled factor's [j][k].
-    float Areg[16];
+    const int j = lane & 15;  // path column of the tile; also the A-operand row
+    const int g = lane >> 4;  // k group fed by this lane
+    const bool hi = a.d > 16;  // asset block 1 and the k >= 16 steps carry terms
+    const bool full = a.full != 0;
+    // A operands of steps 0..7; this lane supplies the scaled factor's
+    // [16 a + j][16 (s/4) + 4 g + s%4].
+    float A0[8], A1[8];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) Areg[s] = Lpad[j * 32 + 8 * (s >> 2) + 4 * h + (s & 3)];
-    // Accumulator register r of this lane holds row i = 8*(r/4) + 4*h + (r%4);
-    // every tile's accumulation starts from base[r] = c0[i] log2 e.
-    v16f base;
-    float ew[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-        base[r] = c0[i];
-        ew[r] = ws[i];
+    for (int s = 0; s < 8; ++s) {
+        A0[s] = Lpad[j * 32 + 16 * (s >> 2) + 4 * g + (s & 3)];
+        A1[s] = Lpad[(16 + j) * 32 + 16 * (s >> 2) + 4 * g + (s & 3)];
     }
-    const int kblk = (a.d + 7) >> 3;  // k blocks of 8 actually used
+    // Accumulator register r of block a holds asset 16 a + 4 g + r; every
+    // tile's accumulation starts from c0 log2 e of that asset.
+    v4f base0, base1;
+    float ew0[4], ew1[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        base0[r] = c0[4 * g + r];
+        base1[r] = c0[16 + 4 * g + r];
+        ew0[r] = ws[4 * g + r];
+        ew1[r] = ws[16 + 4 * g + r];
+    }
     double s = 0.0, s2 = 0.0;
-    const int64_t ntiles = (a.N + 31) >> 5;
+    const int64_t ntiles = (a.N + 15) >> 4;
     const int64_t wave = ((int64_t)blk * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)nblk * blockDim.x) >> 6;
     for (int64_t t = wave; t < ntiles; t += nwaves) {
-        const int64_t p = (t << 5) + j;
-        v16f acc = base;
+        const int64_t p = (t << 4) + j;
+        float z[4];
+        pcf::normals4<float>(a.seed, PCF_STREAM_BASKET, (uint64_t)p, (uint32_t)g, z);
+        v4f acc0 = base0, acc1 = base1;
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-            if (cc < 2 || cc < kblk) {  // d >= 16: blocks 0 and 1 always run
-                float z[4];
-                pcf::normals4<float>(a.seed, PCF_STREAM_BASKET, (uint64_t)p, (uint32_t)(2 * cc + h), z);
+        for (int q = 0; q < 4; ++q) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0[q], z[q], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[q], z[q], acc1, 0, 0, 0);
+        }
+        if (hi) {
+            pcf::normals4<float>(a.seed, PCF_STREAM_BASKET, (uint64_t)p, (uint32_t)(4 + g), z);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[4 + q], z[q], acc1, 0, 0, 0);
+            if (full)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Areg[4 * cc + q], z[q], acc, 0, 0, 0);
-            }
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A0[4 + q], z[q], acc0, 0, 0, 0);
         }
         float part = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part = fmaf(ew[r], pcf::exp2_f32(acc[r]), part);
+        for (int r = 0; r < 4; ++r) part = fmaf(ew0[r], pcf::exp2_f32(acc0[r]), part);
+        if (hi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part = fmaf(ew1[r], pcf::exp2_f32(acc1[r]), part);
+        // the 32 assets of path j sit in the four lane groups
+        part += __shfl_xor(part, 16, 64);
         const float basket = part + __shfl_xor(part, 32, 64);
         const float v = pcf::payoff<float>(basket, a.Ef, a.pff);
-        if (h == 0 && p < a.N) {
+        if (g == 0 && p < a.N) {
             const double dv = (double)v;
             s += dv;
             s2 += dv * dv;
@@ -231,8 +258,8 @@ __device__ __forceinline__ void basket_mfma_body(const BasketArgs& a, const floa
         pcf::write_moments<1>(tot, a.N, out);
 }
 
-// Four waves per SIMD: the compiler then fits the kernel in 106 VGPRs and
-// no AGPRs without spills (left to itself it took 102 VGPRs + 32 AGPRs, 3 waves).
+// At least four waves per SIMD: the operands and accumulators then stay in
+// VGPRs (86, no AGPRs, no spills: five waves).
 __global__ __launch_bounds__(kBasketThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void k_basket_mfma(const BasketJob* __restrict__ jobs, int njobs) {
     __shared__ pcf::CSum lds[(kBasketThreads / pcf::kWave) * 2];
@@ -469,10 +496,11 @@ int basket_prepare(pcf_ctx* ctx, const pcf_request& q, BasketPrep& p) {
     a.pff = (float)q.payoff_fun;
     a.N = q.N;
     a.d = d;
+    a.full = p.full ? 1 : 0;
     a.seed = o.seed;
     // Built-in shape: 256 threads, at most 16 blocks per CU (the MFMA kernel's
-    // items are 32-path tiles, one wavefront per tile at a time).
-    const int64_t ntiles = (q.N + 31) / 32;
+    // items are 16-path tiles, one wavefront per tile at a time).
+    const int64_t ntiles = (q.N + 15) / 16;
     const int64_t max_blocks = 16LL * (ctx->cu_count > 0 ? ctx->cu_count : 256);
     int B = 256;
     int ipt = p.mfma ? pcf_capped_ipt(ntiles, B / 64, 2, max_blocks) : pcf_capped_ipt(q.N, B, 1, max_blocks);

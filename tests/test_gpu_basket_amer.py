@@ -115,6 +115,30 @@ def test_basket_fp32_vs_fp64(engine, d, variant):
     assert abs(r32.price - r64.price) < 0.05 * r64.std_err
 
 
+@pytest.mark.parametrize("d", [24, 32])
+def test_basket_mfma_full_factor(engine, d):
+    """Assets i and i + d/2 perfectly correlated, the rest independent: the
+    Cholesky factorisation fails at pivot d/2 and the eigen fallback's factor
+    puts the pairs' common factor in columns d/2..d-1 of rows 0..d/2-1 (the
+    Jacobi rotation of each pair leaves its eigenvalue 2 at the higher index).
+    Rows below 16 then have terms at k >= 16, which the MFMA kernel's
+    triangular form skips: FP32 MFMA must match FP32 FMA and FP64 on the
+    same normals."""
+    say("basket MFMA, full factor", d)
+    N = 1 << 20
+    h = d // 2
+    corr = np.eye(d)
+    for i in range(h):
+        corr[i, i + h] = corr[i + h, i] = 1.0
+    kw = dict(corr=corr, seed=7)
+    r64 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.0, precision="fp64", **kw)
+    r32 = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.0, kernel_variant=2, **kw)
+    rfma = engine.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", d, 0.0, kernel_variant=1, **kw)
+    assert r64.lsm_fallback == 1 and r32.lsm_fallback == 1
+    assert abs(r32.price - r64.price) < 0.05 * r64.std_err
+    assert abs(r32.price - rfma.price) < 0.05 * r64.std_err
+
+
 def test_basket_2p24_paths_full_size(engine):
     """d = 8 and 32, 2^24 paths, sigma_i ~ U[0.15,0.35], rho = 0.3 and a random SPD matrix."""
     N = 1 << 24
