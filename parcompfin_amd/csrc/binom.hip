@@ -62,6 +62,35 @@ __device__ __forceinline__ double term_value(const BinomArgs& a, double comb_val
 // outstanding and their prefetch cannot run deeper than one batch; LDS reads
 // complete in order.
 constexpr int kStageChunk = 1024;
+
+// acc - v of lane 16 r + K, for every lane of row r (r = 0..3), as one
+// v_fmac_f64 whose first source is read through the DPP row broadcast:
+// acc + v * -1 rounds exactly as acc - v.  Only VALU operands: the value
+// comes from an LDS load, which the compiler's wait counts cover.
+#define PCF_SUB_BCAST(K)                                                                   \
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #K " row_mask:0xf bank_mask:0xf" \
+                 : "+v"(acc)                                                                \
+                 : "v"(v), "v"(neg1))
+// The 16 values of each row, subtracted in lane order.
+__device__ __forceinline__ void sub_row16(double& acc, double v, double neg1) {
+    PCF_SUB_BCAST(0);
+    PCF_SUB_BCAST(1);
+    PCF_SUB_BCAST(2);
+    PCF_SUB_BCAST(3);
+    PCF_SUB_BCAST(4);
+    PCF_SUB_BCAST(5);
+    PCF_SUB_BCAST(6);
+    PCF_SUB_BCAST(7);
+    PCF_SUB_BCAST(8);
+    PCF_SUB_BCAST(9);
+    PCF_SUB_BCAST(10);
+    PCF_SUB_BCAST(11);
+    PCF_SUB_BCAST(12);
+    PCF_SUB_BCAST(13);
+    PCF_SUB_BCAST(14);
+    PCF_SUB_BCAST(15);
+}
+#undef PCF_SUB_BCAST
 __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* __restrict__ tab,
                                                      const double* __restrict__ pre,
                                                      pcf::CSum* __restrict__ partials,
@@ -79,6 +108,8 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
     double acc = pre[k1 < 0 ? 0 : (k1 > a.N ? a.N : k1)];
     // Part 2: j = 2 .. N-i.  Unmasked while j <= N-(i0+63).
     {
+        const int lo = (int)(threadIdx.x & 15);
+        const double neg1 = -1.0;
         __shared__ double stage[2][kStageChunk];
         const int b0 = __builtin_amdgcn_readfirstlane((int)((int64_t)blockIdx.x * blockDim.x));
         const int blk_hi = N - b0;                   // largest j of any lane of the block
@@ -108,9 +139,18 @@ __global__ __launch_bounds__(256) void k_binom_chain(BinomArgs a, const double* 
             const int last = base + kStageChunk - 1;
             const int e_all = last < hi_all ? last : hi_all;
             int j = base;
-            for (; j + 15 <= e_all; j += 16) {
-#pragma unroll
-                for (int t = 0; t < 16; ++t) acc -= S[j - base + t];
+            // Groups of 16: lane l loads entry j + l%16 (each 16-lane row holds
+            // the same 16 values) and subtracts the row's 16 values in order,
+            // one broadcast DPP instruction each; the next group loads meanwhile.
+            if (j + 15 <= e_all) {
+                double vcur = S[j - base + lo];
+                for (; j + 15 <= e_all; j += 16) {
+                    // unconditional (clamped into the chunk; unused past e_all)
+                    const int nx = j + 16 - base + lo;
+                    const double vnext = S[nx < kStageChunk ? nx : kStageChunk - 1];
+                    sub_row16(acc, vcur, neg1);
+                    vcur = vnext;
+                }
             }
             for (; j <= e_all; ++j) acc -= S[j - base];
             const int e_any = last < hi_any ? last : hi_any;
