@@ -71,8 +71,12 @@ constexpr int kStageChunk = 1024;
     asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #K " row_mask:0xf bank_mask:0xf" \
                  : "+v"(acc)                                                                \
                  : "v"(v), "v"(neg1))
-// The 16 values of each row, subtracted in lane order.
+// The 16 values of each row, subtracted in lane order.  A DPP read of a VGPR
+// needs two wait states after a VALU write of it (v may have just been moved
+// into place), which the compiler does not see inside the assembly text: the
+// s_nop 1 provides them.
 __device__ __forceinline__ void sub_row16(double& acc, double v, double neg1) {
+    asm volatile("s_nop 1");
     PCF_SUB_BCAST(0);
     PCF_SUB_BCAST(1);
     PCF_SUB_BCAST(2);
