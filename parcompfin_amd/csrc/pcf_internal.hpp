@@ -134,10 +134,9 @@ int pcf_check_opts(pcf_ctx* ctx, const pcf_opts* o);
 // Copies `count` FP64 normals to ctx->normals.
 int pcf_upload_normals(pcf_ctx* ctx, const pcf_opts* o, int64_t count);
 
-// Waits for an event by polling it from this thread.  A blocking wait sleeps
-// in the driver until the completion interrupt and wakes a few hundred
-// microseconds after the work ended (measured between back-to-back served
-// requests); polling hands the next request to the device without that delay.
+// Waits for an event by polling it from this thread (yielding, then every
+// 10 us).  A blocking wait sleeps in the driver until the completion
+// interrupt and can wake a few hundred microseconds after the work ended.
 hipError_t pcf_wait_event(hipEvent_t ev);
 
 // Records the begin event, returns status.  Timing: begin/kbeg/kend/end.
