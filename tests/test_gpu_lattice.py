@@ -50,6 +50,29 @@ def test_lattice_other_parameters_bitwise(engine):
                 assert got == want, (args, american, pf)
 
 
+@pytest.mark.parametrize("american", [0, 1])
+def test_lattice_tiny_values_bitwise(engine, american):
+    """Far out-of-the-money calls whose values run down through the range
+    where the division by R is rescaled (below 2^-900) and into subnormals
+    (where it falls back to the division instruction): the root itself is
+    1e-288 .. 1e-323, so any node's last bit would show in it."""
+    for E in (3e6, 3.5e6, 4e6, 4.5e6, 5e6):
+        say("lattice tiny values", E, "american" if american else "european")
+        got = engine.binom_vanilla(100, E, 0.05, 0.3, 1.0, 2000, "call", american=bool(american)).price
+        want = O.lib().orc_binom_vanilla(100, E, 0.05, 0.3, 1.0, 2000, 1.0, american)
+        assert got == want, (E, got, want)
+
+
+@pytest.mark.parametrize("american", [0, 1])
+def test_lattice_n1e4_bitwise(engine, american):
+    for args in ((100, 100, 0.06, 0.2, 1.0), (100, 110, 0.02, 0.75, 1.0)):
+        for pf, code in (("call", 1.0), ("put", -1.0)):
+            say("lattice N=1e4", args, pf, american)
+            got = engine.binom_vanilla(*args, 10000, pf, american=bool(american)).price
+            want = O.lib().orc_binom_vanilla(*args, 10000, code, american)
+            assert got == want, (args, pf, got, want)
+
+
 def test_lattice_vs_reference_fixtures(engine):
     rows = [r for r in csv.DictReader(open(os.path.join(HERE, "golden", "ref_kernels.csv")))
             if r["prog"].startswith("binom_vanilla")]
