@@ -1116,7 +1116,7 @@ constexpr size_t lsm_lds_row_bytes(int threads) {
 // Copies the owned values of date m (store row m-1) into rowbuf parity m & 1.
 // Lanes whose pair index is past N/2 copy nothing (their entries are never
 // read), so every load stays inside the row.  The copies are non-temporal
-// (each row is read once): A/B on one box, bits unchanged, 2^21-2^22 paths
+// (each row is read once): A/B over two runs, bits unchanged, 2^21-2^22 paths
 // 2-3% faster (FP32 2^22 0.776 -> 0.758 ms), smaller sizes unchanged.
 template <typename Real, int PMAX>
 __device__ __forceinline__ void lsm_stage_row(Real* rowbuf, const Real* __restrict__ store, int m, int64_t N,
