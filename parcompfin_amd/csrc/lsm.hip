@@ -1690,12 +1690,14 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 // pairs with latency exposed at few waves, so the grid targets 64 blocks for
 // up to 2^13 antithetic pairs, 128 up to 2^15 and one per CU beyond, and the
 // pairs of a block are spread over as many threads as a block holds (1024)
-// before a thread takes more than one pair (1, 2, 4, 8).
+// before a thread takes more than one pair: the fewest pairs per thread (1..8)
+// that fit, not only powers of two (the kernel's register budget is the next
+// power of two; the pair slots past the count hold no path).
 void lsm_persistent_shape(int64_t half, int cus, int* bs, int* pm) {
     const int64_t target = half <= (1 << 13) ? 64 : half <= (1 << 15) ? 128 : std::max(cus, 64);
     const int64_t per_block = (half + target - 1) / target;
     int p = 1;
-    while (p < 8 && (per_block + p - 1) / p > 1024) p <<= 1;
+    while (p < 8 && (per_block + p - 1) / p > 1024) ++p;
     const int64_t b = ((per_block + p - 1) / p + 63) / 64 * 64;
     *pm = p;
     *bs = (int)std::min<int64_t>(1024, std::max<int64_t>(64, b));
@@ -1939,9 +1941,10 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                             "persistent LSM shape %d x %d: need a block size in 64..1024 (multiple of 64) and "
                             "1..8 pairs per thread", bs, pm);
-        int p2 = 1;
+        int p2 = 1;  // register budget: pair slots per thread, a power of two
         while (p2 < pm) p2 <<= 1;
-        const int64_t g = (P.half + (int64_t)p2 * bs - 1) / ((int64_t)p2 * bs);
+        // pm pairs per thread: slots pm..p2-1 hold no path anywhere
+        const int64_t g = (P.half + (int64_t)pm * bs - 1) / ((int64_t)pm * bs);
         int gmax;
         if (p2 == 1) gmax = max_coresident<1>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 2) gmax = max_coresident<2>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
