@@ -959,9 +959,9 @@ __device__ __forceinline__ double block_sum8(const double v[8], double* lds) {
 //     complete before the block's arrival is counted;
 //   lsm_wait_all: wave 0 polls every arrival counter until each has counted
 //     its blocks `gen` times (dates completed including this one);
-//   lsm_gather: every block reads all partials with coherent loads and sums
-//     them in the same fixed order, so g (the grid totals, in every thread)
-//     is bit-identical in every block.
+//   lsm_gather_solve: every block reads all partials with coherent loads and
+//     sums them in the same fixed order, so the grid totals are bit-identical
+//     in every block, and solves the normal equations from them.
 __device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ slots, const LsmSync& s) {
     if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
         const int lane = threadIdx.x;
@@ -997,8 +997,25 @@ __device__ __forceinline__ void lsm_wait_all(const LsmSync& s, unsigned int gen)
     __syncthreads();
 }
 
-__device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict__ slots, double* red_b,
-                                           double* tot_sh) {
+// 64-bit readlane: lane l's value of v, uniform over the wave.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// lsm_gather then the solve of the normal equations, once per block.  Wave 0
+// forms the 8 totals (threads 0..7, waves in order), takes all of them from
+// those lanes, solves, and hands the coefficients to the block through LDS;
+// the other waves wait at the barrier.  Had every thread solved, the waves
+// sharing a SIMD (4 at 1024 threads) would each issue the same dependent
+// FP64 chain (square roots, reciprocals) in turn.  The solve is a pure
+// function of the totals, which are the same bits in every block, so every
+// block still derives the same coefficients.
+template <bool REF>
+__device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __restrict__ slots, double* red_b,
+                                                 double* tot_sh) {
     // Only the first min(grid, block) threads fetch; the waves holding them
     // sum in the fixed order of block_sum8 (wave trees, then waves in order),
     // the others stay out.
@@ -1016,14 +1033,23 @@ __device__ __forceinline__ void lsm_gather(double g[8], const double* __restrict
         if ((lane & 7) == 0) red_b[wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
     }
     __syncthreads();
-    if (threadIdx.x < 8) {
-        double tot = red_b[threadIdx.x];
-        for (int w = 1; w < nw; ++w) tot += red_b[w * 8 + threadIdx.x];
-        tot_sh[threadIdx.x] = tot;
+    if (wave == 0) {  // the whole wave (block sizes are multiples of 64)
+        double tot = 0.0;
+        if (lane < 8) {
+            tot = red_b[lane];
+            for (int w = 1; w < nw; ++w) tot += red_b[w * 8 + lane];
+        }
+        double g[8], c[4];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] = readlane_f64(tot, k);
+        lsm_solve_g<REF>(g, c);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tot_sh[k] = c[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) g[k] = tot_sh[k];
+    for (int k = 0; k < 4; ++k) cm[k] = tot_sh[k];
 }
 
 // The same three steps for 8C sums (C chunks of 8), used by the K-term basis:
@@ -1065,9 +1091,11 @@ __device__ __forceinline__ void lsm_publish_chunks(const double t[C], double* __
     }
 }
 
-template <int C>
-__device__ __forceinline__ void lsm_gather_chunks(double* g, const double* __restrict__ slots, double* red_b,
-                                                  double* tot_sh) {
+// The chunked gather and the K-term solve, once per block as in
+// lsm_gather_solve; the coefficients (K + 1 <= 8C values) go through tot_sh.
+template <int C, int K>
+__device__ __forceinline__ void lsm_gather_solve_chunks(double cm[K + 1], const double* __restrict__ slots,
+                                                        double* red_b, double* tot_sh) {
     const unsigned int active = gridDim.x < blockDim.x ? gridDim.x : blockDim.x;
     const int nw = (int)((active + pcf::kWave - 1) / pcf::kWave);
     const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
@@ -1086,16 +1114,27 @@ __device__ __forceinline__ void lsm_gather_chunks(double* g, const double* __res
         }
     }
     __syncthreads();
-    if (threadIdx.x < 8)
+    static_assert(K + 1 <= 8 * C, "coefficients pass through tot_sh");
+    if (wave == 0) {
+        double g[8 * C], c[K + 1];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            double tot = red_b[128 * c + threadIdx.x];
-            for (int w = 1; w < nw; ++w) tot += red_b[128 * c + w * 8 + threadIdx.x];
-            tot_sh[8 * c + threadIdx.x] = tot;
+        for (int cc = 0; cc < C; ++cc) {
+            double tot = 0.0;
+            if (lane < 8) {
+                tot = red_b[128 * cc + lane];
+                for (int w = 1; w < nw; ++w) tot += red_b[128 * cc + w * 8 + lane];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[8 * cc + k] = readlane_f64(tot, k);
         }
+        lsm_solve_poly<K>(g, c);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k <= K; ++k) tot_sh[k] = c[k];
+    }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 8 * C; ++k) g[k] = tot_sh[k];
+    for (int k = 0; k <= K; ++k) cm[k] = tot_sh[k];
 }
 
 // Row staging in LDS.  With kLdsRow, the path values of two dates live in the
@@ -1380,9 +1419,8 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
         }
         lsm_wait_all(sy, gen);
         if constexpr (K == 3) {
-            double g[8], cm[4];
-            lsm_gather(g, slots_m, red_b, tot_sh);
-            lsm_solve_g<REF>(g, cm);  // every thread, same totals -> same coefficients
+            double cm[4];
+            lsm_gather_solve<REF>(cm, slots_m, red_b, tot_sh);
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
@@ -1409,9 +1447,8 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
                 }
             }
         } else {
-            double g[8 * Bs::kChunks], cm[K + 1];
-            lsm_gather_chunks<Bs::kChunks>(g, slots_m, red_b, tot_sh);
-            lsm_solve_poly<K>(g, cm);  // every thread, same totals -> same coefficients
+            double cm[K + 1];
+            lsm_gather_solve_chunks<Bs::kChunks, K>(cm, slots_m, red_b, tot_sh);
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k <= K; ++k) b.coef[Bs::kCoef * m + k] = cm[k];
