@@ -118,12 +118,27 @@ def serve_once(srv, mix, precision):
 def serve_steps(srv, mix, precision, K):
     """K steps under sustained load: the K sets are submitted back to back
     (the server keeps working while later sets arrive), then every result is
-    waited for.  Returns the elapsed time and the last set's results."""
+    waited for.  Returns the elapsed time and every set's results."""
     t0 = time.perf_counter()
     tickets = [[srv.submit(q["kind"], *q["args"], seed=q["seed"], precision=precision, **q["kw"]) for q in mix]
                for _ in range(K)]
     res = [[srv.wait(t) for t in ts] for ts in tickets]
-    return time.perf_counter() - t0, res[-1]
+    return time.perf_counter() - t0, res
+
+
+def kernel_spread(mix, sets, kind="amer"):
+    """Spread of each `kind` request's kernel time over the sustained sets:
+    max/min per request (the worst and the median over requests), and the
+    summed kernel time per set against the sum of the per-request minima."""
+    import numpy as np
+    idx = [i for i, q in enumerate(mix) if q["kind"] == kind]
+    if not idx or len(sets) < 2:
+        return None
+    kt = np.array([[s[i].kernel_ms for i in idx] for s in sets])
+    ratio = kt.max(axis=0) / kt.min(axis=0)
+    return {"requests": len(idx), "sets": len(sets), "worst_max_over_min": float(ratio.max()),
+            "median_max_over_min": float(np.median(ratio)), "kernel_ms_per_set": float(kt.sum(axis=1).mean()),
+            "sum_of_min_kernel_ms": float(kt.min(axis=0).sum())}
 
 
 def poisson_phase(srv, mix, rate, precision="fp32k"):
@@ -542,21 +557,23 @@ def main():
     mix = make_mix(2026, args.requests)
     ps = path_steps(mix)
     srv = Server(eng, mem_budget_bytes=16 << 30, max_inflight=args.slots)
-    results, sstats, closed = {}, {}, {}
+    results, sstats, closed, spread = {}, {}, {}, {}
     for prec in ("fp32k", "fp64"):
         for _ in range(W):
             serve_once(srv, mix, prec)
         # closed loop: one set at a time (reported beside the sustained figure)
         closed[prec] = 1e3 * sum(serve_once(srv, mix, prec)[0] for _ in range(K)) / K
         # sustained load: the timed K steps
-        t, rs = serve_steps(srv, mix, prec, K)
+        t, sets = serve_steps(srv, mix, prec, K)
+        rs = sets[-1]
         results[prec] = ([t], rs)
+        spread[prec] = kernel_spread(mix, sets)
         # cumulative server counters after this precision's steps
         sstats[prec] = {k: v for k, v in srv.stats().items()
                         if k in ("evictions", "underestimates", "peak_held_bytes", "peak_charged_bytes",
                                  "budget_bytes", "completed")}
         log(f"mix {prec}: {1e3 * t / K:.2f} ms/step sustained ({closed[prec]:.2f} closed loop), "
-            f"{ps * K / t:.4g} paths·steps/s")
+            f"{ps * K / t:.4g} paths·steps/s; American kernel spread {spread[prec]}")
     t32, r32 = results["fp32k"]
     t64, r64 = results["fp64"]
     value = ps * K / sum(t32)
@@ -631,6 +648,7 @@ def main():
         "latency": latency,
         "mix_by_kind": by_kind,
         "server_stats": sstats,
+        "american_kernel_spread": spread,
         "per_config": configs,
         "autotune": tuned,
         "device": info["name"],

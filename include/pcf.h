@@ -127,6 +127,11 @@ typedef struct pcf_result {
 typedef struct pcf_ctx pcf_ctx;
 
 /* ---- context ------------------------------------------------------------ */
+/* A context owns one HIP stream and its resident buffers; use it from one
+ * thread at a time.  Contexts of one process on one device share the
+ * ordering of mc_amer's persistent launches: each runs after the kernels the
+ * other contexts had enqueued, and the other contexts' heavy launches
+ * (about 40 us of work and up) run after the last persistent launch. */
 int pcf_ctx_create(int device, pcf_ctx** out);
 int pcf_ctx_destroy(pcf_ctx* ctx);
 const char* pcf_ctx_last_error(const pcf_ctx* ctx);

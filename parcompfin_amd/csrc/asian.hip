@@ -338,6 +338,9 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
     }
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(AsiaJob) * nl, hipMemcpyHostToDevice,
                                       ctx->stream));
+    double work = 0.0;
+    for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N * (double)reqs[live[k]]->M;
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (mode == kFp32k) {
         if (control) launch_asia<kFp32k, true>(ctx, boff, B, nl);

@@ -639,6 +639,9 @@ int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_resu
     }
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(BasketJob) * nl, hipMemcpyHostToDevice,
                                       ctx->stream));
+    double work = 0.0;
+    for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N * (double)reqs[live[k]]->assets;
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, p0.mode == kFp32k ? work : kPersistentGateFp64 * work));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (p0.mfma)
         k_basket_mfma<<<boff, p0.B, 0, ctx->stream>>>(ctx->jobs.as<BasketJob>(), nl);

@@ -341,6 +341,7 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     if (chain && (st = ensure_logtab(ctx, N))) return st;
     if (!chain && (st = ensure_logfact(ctx, N))) return st;
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
         k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),

@@ -4,10 +4,36 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <thread>
+#include <vector>
 
 #include "pcf_internal.hpp"
+
+namespace {
+// Every live context, so that a persistent launch can be ordered after the
+// kernels that other contexts on its device have already enqueued.
+std::mutex& ctx_registry_mu() {
+    static std::mutex m;
+    return m;
+}
+std::vector<pcf_ctx*>& ctx_registry() {
+    static std::vector<pcf_ctx*> v;
+    return v;
+}
+}  // namespace
+
+hipError_t pcf_wait_other_contexts(const pcf_ctx* self, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(ctx_registry_mu());
+    for (pcf_ctx* c : ctx_registry()) {
+        if (c == self || c->device != self->device) continue;
+        const hipError_t e = hipStreamWaitEvent(s, c->ev_kend, 0);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 int pcf_fail(pcf_ctx* ctx, int code, const char* fmt, ...) {
     if (ctx) {
@@ -83,12 +109,21 @@ int pcf_ctx_create(int device, pcf_ctx** out) {
     if ((e = hipHostMalloc(&ctx->h_jobs, kMaxBatch * kJobBytes, 0)) != hipSuccess)
         return bail(e, "hipHostMalloc jobs");
     if ((e = ctx->jobs.ensure(kMaxBatch * kJobBytes)) != hipSuccess) return bail(e, "alloc jobs");
+    {
+        std::lock_guard<std::mutex> lk(ctx_registry_mu());
+        ctx_registry().push_back(ctx);
+    }
     *out = ctx;
     return PCF_OK;
 }
 
 int pcf_ctx_destroy(pcf_ctx* ctx) {
     if (!ctx) return PCF_OK;
+    {
+        std::lock_guard<std::mutex> lk(ctx_registry_mu());
+        auto& v = ctx_registry();
+        v.erase(std::remove(v.begin(), v.end(), ctx), v.end());
+    }
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     pcf_lsm_release(ctx);

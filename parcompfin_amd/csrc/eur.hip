@@ -203,6 +203,9 @@ int pcf_eur_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result*
         boff += (unsigned int)p.nblk;
     }
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(EurJob) * nl, hipMemcpyHostToDevice, ctx->stream));
+    double work = 0.0;
+    for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N;
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     const EurJob* dj = ctx->jobs.as<EurJob>();
     if (mode == kFp32k) k_eur<kFp32k><<<boff, B, 0, ctx->stream>>>(dj, nl);

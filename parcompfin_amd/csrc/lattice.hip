@@ -200,6 +200,7 @@ extern "C" int pcf_binom_vanilla(pcf_ctx* ctx, double S0, double E, double r, do
         ctx->lattice_ptr = (void*)up;
     }
 
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     const int lgrid = (int)std::min<int64_t>(4096, ((int64_t)row + kLatThreads - 1) / kLatThreads);
     if (american) k_lat_leaves<true><<<lgrid, kLatThreads, 0, ctx->stream>>>(P, up, dn, A);

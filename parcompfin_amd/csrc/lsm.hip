@@ -1530,9 +1530,10 @@ hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStr
     void* args[] = {&a};
     // A plain launch: the grid fits the device's co-resident capacity (checked
     // by the caller), persistent launches run one after another on one stream
-    // per device, and every other kernel of the library finishes on its own,
-    // so all blocks become resident -- at worst after a co-running kernel
-    // ends.  Every wait is bounded all the same.  ev0 / ev1 take the kernel's
+    // per device, each after the kernels other contexts had enqueued, and
+    // every other kernel of the library finishes on its own, so all blocks
+    // become resident -- at worst after a co-running kernel ends.  Every wait
+    // is bounded all the same.  ev0 / ev1 take the kernel's
     // own start and end times (no marker packets between consecutive launches).
     return hipExtLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, dim3((unsigned)grid),
                               dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s, ev0, ev1, 0);
@@ -1674,6 +1675,19 @@ hipError_t coop_stream(int device, hipStream_t* out) {
 // Serialises enqueueing onto that stream (launch + its bracketing events).
 std::mutex& persistent_mu() {
     static std::mutex m;
+    return m;
+}
+// Per device: an event recorded after every persistent launch (under
+// persistent_mu), and whether it has been recorded yet.  Heavy kernels of
+// other kinds wait for it (pcf_after_persistent): started while a persistent
+// grid waits for CUs, a long Asian launch kept taking the CUs its blocks
+// freed, and the grid's resident blocks spun for up to its whole duration.
+struct PersistEnd {
+    hipEvent_t ev = nullptr;
+    bool recorded = false;
+};
+std::map<int, PersistEnd>& persist_end() {
+    static std::map<int, PersistEnd> m;
     return m;
 }
 
@@ -2070,6 +2084,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         {
             std::lock_guard<std::mutex> pl(persistent_mu());
             if (handover) PCF_HIP_CHECK(ctx, hipStreamWaitEvent(cs, ctx->ev_ready, 0));
+            PCF_HIP_CHECK(ctx, pcf_wait_other_contexts(ctx, cs));
             hipEvent_t e0 = ctx->ev_kbeg, e1 = ctx->ev_kend;
             hipError_t le;
             if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
@@ -2077,6 +2092,10 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
+            PersistEnd& pe = persist_end()[ctx->device];
+            if (!pe.ev) PCF_HIP_CHECK(ctx, hipEventCreateWithFlags(&pe.ev, hipEventDisableTiming));
+            PCF_HIP_CHECK(ctx, hipEventRecord(pe.ev, cs));
+            pe.recorded = true;
         }
         PCF_HIP_CHECK(ctx, pcf_wait_event(ctx->ev_kend));
         PCF_HIP_CHECK(ctx, pcf_wait_event(ctx->ev_begin));
@@ -2138,6 +2157,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         exec = g.exec;
     }
     if (!persistent) {
+        PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL));
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
         PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
@@ -2204,3 +2224,11 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
     return PCF_OK;
 }
 }  // namespace
+
+hipError_t pcf_after_persistent(pcf_ctx* ctx, double work) {
+    if (work < kPersistentGateWork) return hipSuccess;
+    std::lock_guard<std::mutex> pl(persistent_mu());
+    auto it = persist_end().find(ctx->device);
+    if (it == persist_end().end() || !it->second.recorded) return hipSuccess;
+    return hipStreamWaitEvent(ctx->stream, it->second.ev, 0);
+}

@@ -139,6 +139,25 @@ int pcf_upload_normals(pcf_ctx* ctx, const pcf_opts* o, int64_t count);
 // interrupt and can wake a few hundred microseconds after the work ended.
 hipError_t pcf_wait_event(hipEvent_t ev);
 
+// Ordering around persistent LSM launches, which need all their blocks
+// resident at once.  A persistent launch waits for the last kernel every
+// other context on its device has enqueued (its kernel-end event), and a
+// heavy kernel of any other kind waits for the last persistent launch, so
+// neither kind holds CUs the other's blocks are waiting for.
+//   pcf_wait_other_contexts: stream s (the persistent stream) waits for the
+//     kernel-end event of every other live context on self's device.
+//   pcf_after_persistent: called before a request's kernels are enqueued on
+//     ctx's stream; when `work` (paths x steps of the launch, paths x assets
+//     for a basket, times kPersistentGateFp64 for FP64 arithmetic; HUGE_VAL
+//     for the lattices and the per-date LSM form) reaches
+//     kPersistentGateWork, the stream waits for the last persistent launch on
+//     its device.  2^26 path-steps is about 40 us of the FP32 Asian kernel;
+//     its FP64 form takes about 6 times as long.
+constexpr double kPersistentGateWork = 67108864.0;  // 2^26
+constexpr double kPersistentGateFp64 = 6.0;
+hipError_t pcf_wait_other_contexts(const pcf_ctx* self, hipStream_t s);
+hipError_t pcf_after_persistent(pcf_ctx* ctx, double work);
+
 // Records the begin event, returns status.  Timing: begin/kbeg/kend/end.
 int pcf_timing_begin(pcf_ctx* ctx);
 // Copies `nout` doubles of results to ctx->h_out, waits, fills timing fields.
