@@ -190,6 +190,56 @@ def test_concurrent_persistent_launches_match_lone_calls(engine):
         e.close()
 
 
+def test_persistent_launches_ordered_against_heavy_kernels(engine):
+    """Persistent LSM launches wait for the kernels other contexts had
+    enqueued, and heavy Asian/basket launches wait for the last persistent
+    launch.  Threads on four contexts interleave both kinds (FP32 and FP64)
+    twice over: every call completes and equals its lone call bit for bit."""
+    from parcompfin_amd import Engine
+    amer = [(1 << 18, 50, 600 + s, "fp32k" if s % 2 else "fp64") for s in range(4)] + [(1 << 20, 50, 610, "fp32k")]
+    heavy = [("asia", 1 << 20, 700 + s, "fp32k" if s % 2 else "fp64") for s in range(3)]
+    heavy += [("basket", 1 << 21, 710 + s, "fp32k" if s % 2 else "fp64") for s in range(2)]
+    heavy += [("asia", 1 << 12, 720, "fp32k")]  # below the ordering threshold
+
+    def run_heavy(e, c):
+        kind, N, s, p = c
+        if kind == "asia":
+            return e.mc_asia(100, 100, 0.05, 0.25, 1.0, N, 252, "call", seed=s, precision=p)
+        return e.mc_eur_multi(100, 100, 0.05, 0.2, 1.0, N, "call", 8, 0.3, seed=s, precision=p)
+
+    want_a = [engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision=p) for N, M, s, p in amer]
+    want_h = [run_heavy(engine, c) for c in heavy]
+    engines = [Engine(0) for _ in range(4)]
+    got_a, got_h = [None] * len(amer), [None] * len(heavy)
+
+    def work(k):
+        e = engines[k]
+        if k < 2:
+            for i in range(k, len(amer), 2):
+                N, M, s, p = amer[i]
+                got_a[i] = e.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", seed=s, precision=p)
+        else:
+            for i in range(k - 2, len(heavy), 2):
+                got_h[i] = run_heavy(e, heavy[i])
+
+    say("persistent launches beside heavy kernels, 4 contexts")
+    for _ in range(2):
+        threads = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=120)
+        for c, a, b in zip(amer, want_a, got_a):
+            assert b is not None and a.price == b.price and a.std_err == b.std_err, (c, a.price, b and b.price)
+        for c, a, b in zip(heavy, want_h, got_h):
+            assert b is not None and a.price == b.price and a.std_err == b.std_err, (c, a.price, b and b.price)
+        say("  American kernel ms, lone vs beside:",
+            [(round(a.kernel_ms, 3), round(b.kernel_ms, 3)) for a, b in zip(want_a, got_a)])
+        got_a, got_h = [None] * len(amer), [None] * len(heavy)
+    for e in engines:
+        e.close()
+
+
 def test_served_american_basis_requests(engine):
     """K-term regressions are served like any American request: same result as
     the direct call, alongside quadratic ones."""
