@@ -345,7 +345,7 @@ def basket_roofline(eng, K=10):
             "counters": counters_of("k_basket_mfma")}
 
 
-def profile_one(eng, which, K, W):
+def profile_one(eng, which, K, W, slots=4):
     """One configuration at full size, W warm-up + K timed calls per precision."""
     import numpy as np
     sig8 = np.random.default_rng(8).uniform(0.15, 0.35, size=8)
@@ -375,7 +375,7 @@ def profile_one(eng, which, K, W):
     if which == "mix":
         from parcompfin_amd import Server
         mix = make_mix(2026, 64)
-        with Server(eng, mem_budget_bytes=16 << 30, max_inflight=8) as srv:
+        with Server(eng, mem_budget_bytes=16 << 30, max_inflight=slots) as srv:
             for p in ("fp32k", "fp64"):
                 for _ in range(W):
                     serve_once(srv, mix, p)
@@ -513,7 +513,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=64)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--latency-requests", type=int, default=400)
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time the reference's binom_embar_omp at N = 1e6 once (O(N^2): minutes)")
@@ -540,7 +540,7 @@ def main():
     info = eng.device_info()
     log("device:", info)
     if args.profile:
-        print(json.dumps(profile_one(eng, args.profile, K, W)), flush=True)
+        print(json.dumps(profile_one(eng, args.profile, K, W, args.slots)), flush=True)
         eng.close()
         return
     tuned = []
