@@ -25,7 +25,7 @@
 //    (src/mc_amer.cpp:100; wrong for puts, reproduced for parity), det <= 0
 //    flagged (the reference throws).
 //  * production: basis {1, x, x^2} with x = S/E (same span as {1, S, S^2}),
-//    FP64 sums and a 3x3 Cholesky solve with fallback to {1, x} then {1},
+//    FP64 sums and a 3x3 LDL^T solve with fallback to {1, x} then {1},
 //    exercise test payoff(S, E); FP32 or FP64 path store.
 // Dates with 1-2 in-the-money paths compare payoff with the discounted cash
 // flow directly (src/mc_amer.cpp:75-83), in both forms.
@@ -209,26 +209,29 @@ __device__ __forceinline__ void lsm_solve_g(const double g[8], double c[4]) {
         c[0] = kFlagRegress;
         return;
     }
-    // Production: FP64 Cholesky of [[n,s1,s2],[s1,s2,s3],[s2,s3,s4]] with the
-    // diagonal reciprocals formed once (3 divides instead of 9).
+    // Production: FP64 square-root-free Cholesky, G = L D L^T, of
+    // G = [[n,s1,s2],[s1,s2,s3],[s2,s3,s4]].  Its pivots d1, d2 are the
+    // squares of the Cholesky factor's diagonal, so the fallback rule is the
+    // Cholesky one: basis column j is kept while d_j > 1e-13 G_jj.  Without
+    // square roots the dependent chain is three divisions long; every block
+    // runs it between the date's all-reduce and its decisions.
     const double eps = 1e-13;
-    const double g00 = n, g10 = g[1], g20 = g[2], g11 = g[2], g21 = g[3], g22 = g[4];
+    const double s1 = g[1], s2 = g[2], s3 = g[3], s4 = g[4];
     const double b0 = g[5], b1 = g[6], b2 = g[7];
-    const double l00 = sqrt(g00), i00 = 1.0 / l00;
-    const double l10 = g10 * i00, l20 = g20 * i00;
-    const double p11 = g11 - l10 * l10;
-    const double y0 = b0 * i00;
-    if (p11 > eps * g11) {
-        const double l11 = sqrt(p11), i11 = 1.0 / l11;
-        const double l21 = (g21 - l20 * l10) * i11;
-        const double p22 = g22 - l20 * l20 - l21 * l21;
-        const double y1 = (b1 - l10 * y0) * i11;
-        if (p22 > eps * g22) {
-            const double i22 = 1.0 / sqrt(p22);
-            const double y2 = (b2 - l20 * y0 - l21 * y1) * i22;
-            const double x2 = y2 * i22;
-            const double x1 = (y1 - l21 * x2) * i11;
-            const double x0 = (y0 - l10 * x1 - l20 * x2) * i00;
+    const double i0 = 1.0 / n;
+    const double l10 = s1 * i0, l20 = s2 * i0;
+    const double d1 = fma(-l10, s1, s2);  // s2 - s1^2 / n
+    if (d1 > eps * s2) {
+        const double i1 = 1.0 / d1;
+        const double t21 = fma(-l20, s1, s3);  // l21 d1
+        const double l21 = t21 * i1;
+        const double d2 = fma(-l21, t21, fma(-l20, s2, s4));
+        const double z1 = fma(-l10, b0, b1);  // forward substitution (z0 = b0)
+        if (d2 > eps * s4) {
+            const double z2 = fma(-l21, z1, fma(-l20, b0, b2));
+            const double x2 = z2 / d2;
+            const double x1 = fma(-l21, x2, z1 * i1);
+            const double x0 = fma(-l20, x2, fma(-l10, x1, b0 * i0));
             c[1] = x0;
             c[2] = x1;
             c[3] = x2;
@@ -236,8 +239,8 @@ __device__ __forceinline__ void lsm_solve_g(const double g[8], double c[4]) {
             return;
         }
         // {1, x}
-        const double x1 = y1 * i11;
-        c[1] = (y0 - l10 * x1) * i00;
+        const double x1 = z1 * i1;
+        c[1] = fma(-l10, x1, b0 * i0);
         c[2] = x1;
         c[0] = kFlagReduced;
         return;
@@ -337,6 +340,57 @@ __device__ __forceinline__ void lsm_decide_kept(const LsmParams& P, const double
         when = m;
         cf = pvr;
     }
+}
+
+// The three kept-x passes without a branch per path (used up to 2 pairs per
+// thread), so that a thread's paths interleave -- their LDS reads issued
+// together -- instead of running one exec-masked region after another.  A
+// path out of the money, or not owned (its staged value may be stale),
+// contributes exactly zero: each fused term becomes fma(0, 0, acc) or
+// fma(a, 0, acc) with a finite, i.e. acc + 0, which leaves the non-negative
+// sums unchanged.  The fused forms are those the branchy passes above
+// compile to (acc1 = fma(1/E, S, acc1), acc2 = fma(x, x, acc2), ...,
+// acc5 = fma(disc, cf, acc5), acc6 = fma(x, y, acc6)), so every sum receives
+// the same roundings in the same path order: the same bits.
+template <typename Real>
+__device__ __forceinline__ void lsm_x_keep_flat(const LsmParams& P, bool own, Real S, double acc[5], double& xo) {
+    const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
+    const bool itm = own && pv > 0.0;
+    const double Sd = (double)S;
+    const double x = Sd * P.invE;
+    const double x2 = x * x;
+    const double zS = itm ? Sd : 0.0, zi = itm ? P.invE : 0.0;
+    const double zx = itm ? x : 0.0, zx2 = itm ? x2 : 0.0;
+    acc[0] += itm ? 1.0 : 0.0;
+    acc[1] = fma(zi, zS, acc[1]);
+    acc[2] = fma(zx, zx, acc[2]);
+    acc[3] = fma(zx, zx2, acc[3]);
+    acc[4] = fma(zx2, zx2, acc[4]);
+    xo = itm ? x : -1.0;
+}
+template <typename Real>
+__device__ __forceinline__ void lsm_y_kept_flat(const double* __restrict__ disc, int m, double xo, int32_t when,
+                                                Real cf, double acc[8]) {
+    const bool itm = xo >= 0.0;
+    const double d = itm ? disc[when - m] : 0.0;
+    const double cfd = itm ? (double)cf : 0.0;
+    const double y = d * cfd;
+    acc[5] = fma(d, cfd, acc[5]);
+    acc[6] = fma(xo, y, acc[6]);
+    acc[7] = fma(xo * xo, y, acc[7]);
+}
+// lsm_decide_kept for a date whose flag is small, regress or reduced (the
+// caller tests the flag, which is the same in every thread).
+template <typename Real>
+__device__ __forceinline__ void lsm_decide_kept_flat(const LsmParams& P, const double* __restrict__ disc,
+                                                     const double* __restrict__ cm, bool small, int m, Real S,
+                                                     double xo, int32_t& when, Real& cf) {
+    const Real pvr = pay<Real>(S, (Real)P.E, (Real)P.pf);
+    const double pv = (double)pvr;
+    const double cont = small ? disc[when - m] * (double)cf : cm[1] + xo * (cm[2] + xo * cm[3]);
+    const bool ex = xo >= 0.0 && pv > cont;
+    when = ex ? m : when;
+    cf = ex ? pvr : cf;
 }
 
 // ---- K-term basis {1, x, ..., x^(K-1)}, x = S/E - 1 (K = 2, 4, 5, 6) --------
@@ -953,19 +1007,23 @@ __device__ __forceinline__ double block_sum8(const double v[8], double* lds) {
 
 // Per-date all-reduce of the 8 regression sums, in three steps so that the
 // kernel can work between the arrival and the wait:
-//   lsm_publish: the block's 8 sums (in threads 0..7) go to this date's slots,
-//     sum-major (slots[k * grid + block], so a wave's load of sum k over
-//     consecutive blocks is one contiguous run), with coherent stores that
-//     complete before the block's arrival is counted;
+//   lsm_publish: the block's 8 sums (in threads 0..7) go to this date's slots
+//     with coherent stores that complete before the block's arrival is
+//     counted: sum-major (slots[k * grid + block], so a wave's load of sum k
+//     over consecutive blocks is one contiguous run), or block-major
+//     (slots[8 * block + k]) where the gather lands them in LDS (LAND);
 //   lsm_wait_all: wave 0 polls every arrival counter until each has counted
 //     its blocks `gen` times (dates completed including this one);
-//   lsm_gather_solve: every block reads all partials with coherent loads and
-//     sums them in the same fixed order, so the grid totals are bit-identical
-//     in every block, and solves the normal equations from them.
+//   lsm_gather_solve: every block reads all partials coherently and sums
+//     them in the same fixed order, so the grid totals are bit-identical in
+//     every block, and solves the normal equations from them.
+template <bool LAND>
 __device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ slots, const LsmSync& s) {
     if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
         const int lane = threadIdx.x;
-        if (lane < 8) pcf::publish(slots + (size_t)lane * gridDim.x + blockIdx.x, mine);
+        if (lane < 8)
+            pcf::publish(LAND ? slots + (size_t)blockIdx.x * 8 + lane : slots + (size_t)lane * gridDim.x + blockIdx.x,
+                         mine);
         pcf::stores_complete();
         if (lane == 0)
             __hip_atomic_fetch_add(s.arrive + (blockIdx.x % kArrivalCounters) * kCounterStride, 1u,
@@ -1010,15 +1068,28 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // those lanes, solves, and hands the coefficients to the block through LDS;
 // the other waves wait at the barrier.  Had every thread solved, the waves
 // sharing a SIMD (4 at 1024 threads) would each issue the same dependent
-// FP64 chain (square roots, reciprocals) in turn.  The solve is a pure
+// FP64 chain (reciprocals) in turn.  The solve is a pure
 // function of the totals, which are the same bits in every block, so every
 // block still derives the same coefficients.
-template <bool REF>
+// With LAND (kernels of 4 or 8 pairs per thread), each lane's block published
+// 64 contiguous bytes, and four 16-byte coherent loads per lane land them in
+// LDS (global_load_lds: no registers), all in flight at once.  Through
+// registers, these kernels sit at the 128-VGPR limit and the compiler issued
+// the 8 loads one after another into the same register pair.  Segment s (64
+// blocks) takes land[512 s ..]: load j of lane l writes sums 2j, 2j+1 at
+// land[512 s + 128 j + 2 l].  Grids of the quadratic form hold at most
+// kLandBlocks blocks (checked by the host).
+constexpr int kLandBlocks = 256;
+// Cache-policy operand of those loads: SC1, the bit pcf::fetch's agent-scope
+// atomic loads carry on gfx950, so both read the published sums the same way.
+constexpr int kLoadCoherent = 16;
+template <bool REF, bool LAND>
 __device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __restrict__ slots, double* red_b,
-                                                 double* tot_sh) {
+                                                 double* tot_sh, double* land) {
     // Only the first min(grid, block) threads fetch; the waves holding them
     // sum in the fixed order of block_sum8 (wave trees, then waves in order),
-    // the others stay out.
+    // the others stay out.  Thread t adds blocks t, t + blockDim, ... in that
+    // order on either path.
     const unsigned int active = gridDim.x < blockDim.x ? gridDim.x : blockDim.x;
     const int nw = (int)((active + pcf::kWave - 1) / pcf::kWave);
     const int lane = threadIdx.x & (pcf::kWave - 1), wave = threadIdx.x / pcf::kWave;
@@ -1026,9 +1097,33 @@ __device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __r
         double t[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) t[k] = 0.0;
-        for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+        if constexpr (LAND) {
+            for (unsigned int b0 = (unsigned)wave * pcf::kWave; b0 < gridDim.x; b0 += blockDim.x) {
+                if (b0 + lane < gridDim.x) {
+                    const double* src = slots + (size_t)(b0 + lane) * 8;
+                    double* dst = land + (size_t)(b0 / pcf::kWave) * 512;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
+                    for (int j = 0; j < 4; ++j)
+                        __builtin_amdgcn_global_load_lds((const void*)(src + 2 * j),
+                                                         (__attribute__((address_space(3))) void*)(dst + 128 * j),
+                                                         16, 0, kLoadCoherent);
+                }
+            }
+            pcf::stores_complete();  // this wave's loads have landed
+            for (unsigned int b0 = (unsigned)wave * pcf::kWave; b0 < gridDim.x; b0 += blockDim.x)
+                if (b0 + lane < gridDim.x) {
+                    const double* row = land + (size_t)(b0 / pcf::kWave) * 512 + 2 * lane;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        t[2 * j] += row[128 * j];
+                        t[2 * j + 1] += row[128 * j + 1];
+                    }
+                }
+        } else {
+            for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] += pcf::fetch(slots + (size_t)k * gridDim.x + bb);
+        }
         const double d = wave_sum8(t);
         if ((lane & 7) == 0) red_b[wave * 8 + ((lane >> 3) & 1) + 2 * ((lane >> 4) & 1) + 4 * ((lane >> 5) & 1)] = d;
     }
@@ -1220,6 +1315,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
     __shared__ double red_a[16 * 8 * Bs::kChunks], red_b[16 * 8 * Bs::kChunks], tot_sh[8 * Bs::kChunks];
+    // The per-date gather lands in LDS for the quadratic at 4 or 8 pairs per
+    // thread (lsm_gather_solve).
+    constexpr bool kLand = K == 3 && PMAX >= 4;
+    __shared__ __attribute__((aligned(16))) double land[kLand ? 8 * kLandBlocks : 2];
     __shared__ double disc[kDiscLds];
     extern __shared__ __align__(16) unsigned char lsm_dyn_lds[];  // row staging (kLdsRow)
     // A private copy: the row copies into LDS count as memory writes, and
@@ -1325,6 +1424,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // x-moments were formed last, xc for the date being processed.
     constexpr bool kKeepX = K == 3 && !REF && PMAX <= 4;
     double xc[kKeepX ? 2 * PMAX : 1], xn[kKeepX ? 2 * PMAX : 1];
+    constexpr bool kFlat = kKeepX && PMAX <= 2;  // branch-free passes (lsm_x_keep_flat)
     auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
         for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
@@ -1334,7 +1434,11 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
         }
 #pragma unroll
         for (int k = 0; k < PMAX; ++k)
-            if (t0 + (int64_t)k * T < half) {
+            if constexpr (kFlat) {
+                const bool own = t0 + (int64_t)k * T < half;
+                lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k), xm, xn[2 * k]);
+                lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k + 1), xm, xn[2 * k + 1]);
+            } else if (t0 + (int64_t)k * T < half) {
                 if constexpr (kKeepX) {
                     lsm_accumulate_x_keep<Real>(P, sval(row, m, 2 * k), xm, xn[2 * k]);
                     lsm_accumulate_x_keep<Real>(P, sval(row, m, 2 * k + 1), xm, xn[2 * k + 1]);
@@ -1375,7 +1479,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
 #pragma unroll
             for (int k = 0; k < PMAX; ++k) {
                 const int64_t n = t0 + (int64_t)k * T;
-                if (n < half) {
+                if constexpr (kFlat) {
+                    lsm_y_kept_flat<Real>(disc, m, xc[2 * k], when[2 * k], cf[2 * k], acc);
+                    lsm_y_kept_flat<Real>(disc, m, xc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
+                } else if (n < half) {
                     if constexpr (kKeepX) {
                         lsm_accumulate_y_kept<Real>(disc, m, xc[2 * k], when[2 * k], cf[2 * k], acc);
                         lsm_accumulate_y_kept<Real>(disc, m, xc[2 * k + 1], when[2 * k + 1], cf[2 * k + 1], acc);
@@ -1386,7 +1493,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
                     }
                 }
             }
-            lsm_publish(block_sum8(acc, red_a), slots_m, sy);
+            lsm_publish<kLand>(block_sum8(acc, red_a), slots_m, sy);
         } else {
             double acc[8 * Bs::kChunks];
 #pragma unroll
@@ -1420,7 +1527,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
         lsm_wait_all(sy, gen);
         if constexpr (K == 3) {
             double cm[4];
-            lsm_gather_solve<REF>(cm, slots_m, red_b, tot_sh);
+            lsm_gather_solve<REF, kLand>(cm, slots_m, red_b, tot_sh, land);
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
@@ -1431,7 +1538,12 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
 #pragma unroll
                 for (int k = 0; k < PMAX; ++k) {
                     const int64_t n = t0 + (int64_t)k * T;
-                    if (n < half) {
+                    if constexpr (kFlat) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+                            lsm_decide_kept_flat<Real>(P, disc, cm, (int)cm[0] == kFlagSmall, m, sval(Sc, m, 2 * k + h),
+                                                       xc[2 * k + h], when[2 * k + h], cf[2 * k + h]);
+                    } else if (n < half) {
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             const int i = 2 * k + h;
@@ -1981,12 +2093,12 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         int bs = 0, pm = 0;
         const bool forced = o.block_size > 0 || o.items_per_thread > 0;
         if (forced) {
-            lsm_persistent_shape(P.half, ctx->cu_count, &bs, &pm);
+            lsm_persistent_shape(P.half, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
             if (o.block_size > 0) bs = o.block_size;
             if (o.items_per_thread > 0) pm = o.items_per_thread;
         } else {
             if (K == 3) pcf_tuned_shape(PCF_KIND_AMER, f64 ? 1 : 0, N, &bs, &pm);  // tuned for the quadratic
-            if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, ctx->cu_count, &bs, &pm);
+            if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
         }
         if (bs % 64 || bs < 64 || bs > 1024 || pm < 1 || pm > 8)
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
@@ -2007,6 +2119,8 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
         else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
         if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
+        // The quadratic form gathers at most kLandBlocks blocks' sums through LDS.
+        if (K == 3 && gmax > kLandBlocks) gmax = kLandBlocks;
         if (g >= 1 && g <= gmax) {
             persistent = true;
             pgrid = (int)g;
