@@ -340,7 +340,8 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
                                       ctx->stream));
     double work = 0.0;
     for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N * (double)reqs[live[k]]->M;
-    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work));
+    std::unique_lock<std::mutex> order;  // held until the end event is recorded
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work, &order));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (mode == kFp32k) {
         if (control) launch_asia<kFp32k, true>(ctx, boff, B, nl);
@@ -355,6 +356,7 @@ int pcf_asia_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result
     }
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if (order.owns_lock()) order.unlock();
     pcf_result timing{};
     if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
     for (int k = 0; k < nl; ++k) {

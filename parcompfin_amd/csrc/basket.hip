@@ -641,7 +641,8 @@ int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_resu
                                       ctx->stream));
     double work = 0.0;
     for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N * (double)reqs[live[k]]->assets;
-    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, p0.mode == kFp32k ? work : kPersistentGateFp64 * work));
+    std::unique_lock<std::mutex> order;  // held until the end event is recorded
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, p0.mode == kFp32k ? work : kPersistentGateFp64 * work, &order));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (p0.mfma)
         k_basket_mfma<<<boff, p0.B, 0, ctx->stream>>>(ctx->jobs.as<BasketJob>(), nl);
@@ -653,6 +654,7 @@ int pcf_basket_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_resu
         launch_fma_d<double, kHost>(ctx, boff, p0.B, nl, p0.full, p0.dmax);
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if (order.owns_lock()) order.unlock();
     pcf_result timing{};
     if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
     for (int k = 0; k < nl; ++k) {

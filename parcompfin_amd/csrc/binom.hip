@@ -341,7 +341,8 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
     if (chain && (st = ensure_logtab(ctx, N))) return st;
     if (!chain && (st = ensure_logfact(ctx, N))) return st;
     PCF_HIP_CHECK(ctx, ctx->partials.ensure((size_t)grid * sizeof(pcf::CSum)));
-    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL));
+    std::unique_lock<std::mutex> order;  // held until the end event is recorded
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL, &order));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     if (chain)
         k_binom_chain<<<grid, B, 0, ctx->stream>>>(a, ctx->logtab.as<double>(), ctx->prefix.as<double>(),
@@ -353,6 +354,7 @@ extern "C" int pcf_binom_embar(pcf_ctx* ctx, double S0, double E, double r, doub
                                                ctx->counters.as<unsigned int>(), ctx->out.as<double>());
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if (order.owns_lock()) order.unlock();
     if ((st = pcf_timing_end(ctx, 1, res))) return st;
     // The reference returns exp(-r*T)*result (src/binom_embar.cpp:49).
     res->mean = ctx->h_out[0];

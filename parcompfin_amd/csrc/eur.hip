@@ -205,7 +205,8 @@ int pcf_eur_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result*
     PCF_HIP_CHECK(ctx, hipMemcpyAsync(ctx->jobs.ptr, jobs, sizeof(EurJob) * nl, hipMemcpyHostToDevice, ctx->stream));
     double work = 0.0;
     for (int k = 0; k < nl; ++k) work += (double)reqs[live[k]]->N;
-    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work));
+    std::unique_lock<std::mutex> order;  // held until the end event is recorded
+    PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, mode == kFp32k ? work : kPersistentGateFp64 * work, &order));
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
     const EurJob* dj = ctx->jobs.as<EurJob>();
     if (mode == kFp32k) k_eur<kFp32k><<<boff, B, 0, ctx->stream>>>(dj, nl);
@@ -213,6 +214,7 @@ int pcf_eur_run(pcf_ctx* ctx, int n, const pcf_request* const* reqs, pcf_result*
     else k_eur<kHost><<<boff, B, 0, ctx->stream>>>(dj, nl);
     PCF_HIP_CHECK(ctx, hipGetLastError());
     PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+    if (order.owns_lock()) order.unlock();
     pcf_result timing{};
     if ((st = pcf_timing_end(ctx, kOutPerJob * nl, &timing))) return st;
     for (int k = 0; k < nl; ++k) {

@@ -2271,10 +2271,12 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         exec = g.exec;
     }
     if (!persistent) {
-        PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL));
+        std::unique_lock<std::mutex> order;  // held until the end event is recorded
+        PCF_HIP_CHECK(ctx, pcf_after_persistent(ctx, HUGE_VAL, &order));
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kbeg, ctx->stream));
         PCF_HIP_CHECK(ctx, hipGraphLaunch(exec, ctx->stream));
         PCF_HIP_CHECK(ctx, hipEventRecord(ctx->ev_kend, ctx->stream));
+        if (order.owns_lock()) order.unlock();
         if ((st = stage_readback())) return st;
         if ((st = pcf_timing_end(ctx, 4, res))) return st;
         res->grid_blocks = grid_step;
@@ -2339,10 +2341,12 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
 }
 }  // namespace
 
-hipError_t pcf_after_persistent(pcf_ctx* ctx, double work) {
+hipError_t pcf_after_persistent(pcf_ctx* ctx, double work, std::unique_lock<std::mutex>* hold) {
     if (work < kPersistentGateWork) return hipSuccess;
-    std::lock_guard<std::mutex> pl(persistent_mu());
+    std::unique_lock<std::mutex> pl(persistent_mu());
     auto it = persist_end().find(ctx->device);
-    if (it == persist_end().end() || !it->second.recorded) return hipSuccess;
-    return hipStreamWaitEvent(ctx->stream, it->second.ev, 0);
+    hipError_t e = hipSuccess;
+    if (it != persist_end().end() && it->second.recorded) e = hipStreamWaitEvent(ctx->stream, it->second.ev, 0);
+    if (hold) *hold = std::move(pl);
+    return e;
 }

@@ -152,11 +152,15 @@ hipError_t pcf_wait_event(hipEvent_t ev);
 //     for the lattices and the per-date LSM form) reaches
 //     kPersistentGateWork, the stream waits for the last persistent launch on
 //     its device.  2^26 path-steps is about 40 us of the FP32 Asian kernel;
-//     its FP64 form takes about 6 times as long.
+//     its FP64 form takes about 6 times as long.  With `hold`, a heavy launch
+//     keeps the device's persistent-launch order locked in *hold; the caller
+//     releases it once the kernels' end event (ctx->ev_kend) is recorded, so
+//     a persistent launch enqueued from another thread in between cannot miss
+//     these kernels (it would wait for that context's previous end event).
 constexpr double kPersistentGateWork = 67108864.0;  // 2^26
 constexpr double kPersistentGateFp64 = 6.0;
 hipError_t pcf_wait_other_contexts(const pcf_ctx* self, hipStream_t s);
-hipError_t pcf_after_persistent(pcf_ctx* ctx, double work);
+hipError_t pcf_after_persistent(pcf_ctx* ctx, double work, std::unique_lock<std::mutex>* hold = nullptr);
 
 // Records the begin event, returns status.  Timing: begin/kbeg/kend/end.
 int pcf_timing_begin(pcf_ctx* ctx);
