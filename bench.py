@@ -314,8 +314,9 @@ def lsm_roofline(eng, K=10):
     ach = by / (kms * 1e-3) / 1e9
     traffic = traffic_of("k_lsm_persistent<float>")
     return {"bound": "latency",
-            "limiter": "the per-date grid all-reduce (publish, wait for every block, gather, solve, decide): "
-                       "~6 us per date at 2^20 paths against ~0.5 us of row traffic",
+            "limiter": "per date: each SIMD's issue of the decide, y-terms and x-moments passes over its four "
+                       "waves, then the grid all-reduce (publish, wait for every block, gather, solve); "
+                       "~5.5 us per date at 2^20 paths against ~0.5 us of row traffic",
             "kernel": f"k_lsm_persistent<float> (mc_amer put, 2^20 paths x 50 dates, grid "
                       f"{rs[0].grid_blocks}x{rs[0].block_size})",
             "achieved": ach, "peak": PEAK["hbm"], "unit": "GB/s", "frac": ach / PEAK["hbm"],
