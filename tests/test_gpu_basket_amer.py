@@ -395,6 +395,27 @@ def test_amer_persistent_matches_graph_form(engine, N, M, prec, ref):
     assert g.control_price == p.control_price or abs(g.control_price - p.control_price) < 1e-12 * g.control_price
 
 
+def test_amer_quadratic_persistent_grid_capped_at_256_blocks(engine):
+    """The quadratic's persistent form gathers at most 256 blocks' sums per
+    date through LDS: a forced shape of 512 blocks (64 threads x 1 pair at
+    2^16 paths) is refused for the persistent form and runs as the per-date
+    graph otherwise, with the same decisions as the default shape."""
+    from parcompfin_amd import PcfError
+    N, M = 1 << 16, 50
+    kw = dict(seed=31, precision="fp32k", return_when=True)
+    with pytest.raises(PcfError) as ei:
+        engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", kernel_variant=2, block_size=64, items_per_thread=1,
+                       **kw)
+    assert ei.value.status == 1  # PCF_ERR_INVALID_ARG
+    forced = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", block_size=64, items_per_thread=1, **kw)
+    default = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, N, M, "put", **kw)
+    say("  forced grid", forced.grid_blocks, "x", forced.block_size, "default", default.grid_blocks)
+    assert default.grid_blocks <= 256
+    flips = int((forced.extra["when"] != default.extra["when"]).sum())
+    assert flips <= 2
+    assert abs(forced.price - default.price) <= (1e-12 if flips == 0 else 1e-4) * default.price
+
+
 # ---- K-term regression basis {1, x, .., x^(K-1)}, x = S/E - 1 -------------------
 
 def _amer_poly_ref(N, M, pf, w, K):
