@@ -346,9 +346,9 @@ __device__ __forceinline__ void lsm_decide_kept(const LsmParams& P, const double
 // thread), so that a thread's paths interleave -- their LDS reads issued
 // together -- instead of running one exec-masked region after another.  A
 // path out of the money, or not owned (its staged value may be stale),
-// contributes exactly zero: each fused term becomes fma(0, 0, acc) or
-// fma(a, 0, acc) with a finite, i.e. acc + 0, which leaves the non-negative
-// sums unchanged.  The fused forms are those the branchy passes above
+// contributes exactly zero: its S (x-moments) or discount factor (y-terms) is
+// replaced by 0 before any product, so each fused term becomes fma(a, 0, acc)
+// with a finite, i.e. acc + 0, which leaves the non-negative sums unchanged.  The fused forms are those the branchy passes above
 // compile to (acc1 = fma(1/E, S, acc1), acc2 = fma(x, x, acc2), ...,
 // acc5 = fma(disc, cf, acc5), acc6 = fma(x, y, acc6)), so every sum receives
 // the same roundings in the same path order: the same bits.
@@ -356,24 +356,22 @@ template <typename Real>
 __device__ __forceinline__ void lsm_x_keep_flat(const LsmParams& P, bool own, Real S, double acc[5], double& xo) {
     const double pv = (double)pay<Real>(S, (Real)P.E, (Real)P.pf);
     const bool itm = own && pv > 0.0;
-    const double Sd = (double)S;
+    const double Sd = itm ? (double)S : 0.0;  // then x = x2 = 0 and every term adds 0
     const double x = Sd * P.invE;
     const double x2 = x * x;
-    const double zS = itm ? Sd : 0.0, zi = itm ? P.invE : 0.0;
-    const double zx = itm ? x : 0.0, zx2 = itm ? x2 : 0.0;
-    acc[0] += itm ? 1.0 : 0.0;
-    acc[1] = fma(zi, zS, acc[1]);
-    acc[2] = fma(zx, zx, acc[2]);
-    acc[3] = fma(zx, zx2, acc[3]);
-    acc[4] = fma(zx2, zx2, acc[4]);
+    acc[0] += (double)itm;
+    acc[1] = fma(P.invE, Sd, acc[1]);
+    acc[2] = fma(x, x, acc[2]);
+    acc[3] = fma(x, x2, acc[3]);
+    acc[4] = fma(x2, x2, acc[4]);
     xo = itm ? x : -1.0;
 }
 template <typename Real>
 __device__ __forceinline__ void lsm_y_kept_flat(const double* __restrict__ disc, int m, double xo, int32_t when,
                                                 Real cf, double acc[8]) {
     const bool itm = xo >= 0.0;
-    const double d = itm ? disc[when - m] : 0.0;
-    const double cfd = itm ? (double)cf : 0.0;
+    const double d = itm ? disc[when - m] : 0.0;  // (cf is finite: 0 or a payoff)
+    const double cfd = (double)cf;
     const double y = d * cfd;
     acc[5] = fma(d, cfd, acc[5]);
     acc[6] = fma(xo, y, acc[6]);
