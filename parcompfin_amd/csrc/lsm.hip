@@ -1423,6 +1423,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     constexpr bool kKeepX = K == 3 && !REF && PMAX <= 4;
     double xc[kKeepX ? 2 * PMAX : 1], xn[kKeepX ? 2 * PMAX : 1];
     constexpr bool kFlat = kKeepX && PMAX <= 2;  // branch-free passes (lsm_x_keep_flat)
+    // At 8 pairs (x not kept) the FP32 kernel's x-moments pass is branch-free
+    // too; its other passes, and the FP64 kernel's, would spill.
+    constexpr bool kFlatX8 = K == 3 && !REF && PMAX == 8 && sizeof(Real) == 4;
     auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
         for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
@@ -1436,6 +1439,11 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
                 const bool own = t0 + (int64_t)k * T < half;
                 lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k), xm, xn[2 * k]);
                 lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k + 1), xm, xn[2 * k + 1]);
+            } else if constexpr (kFlatX8) {
+                const bool own = t0 + (int64_t)k * T < half;
+                double xo;  // (not kept)
+                lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k), xm, xo);
+                lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k + 1), xm, xo);
             } else if (t0 + (int64_t)k * T < half) {
                 if constexpr (kKeepX) {
                     lsm_accumulate_x_keep<Real>(P, sval(row, m, 2 * k), xm, xn[2 * k]);
