@@ -1423,6 +1423,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     constexpr bool kKeepX = K == 3 && !REF && PMAX <= 4;
     double xc[kKeepX ? 2 * PMAX : 1], xn[kKeepX ? 2 * PMAX : 1];
     constexpr bool kFlat = kKeepX && PMAX <= 2;  // branch-free passes (lsm_x_keep_flat)
+    // At 4 pairs only the x-moments pass is branch-free: the other two would
+    // spill.
+    constexpr bool kFlatX4 = kKeepX && PMAX == 4;
     // At 8 pairs (x not kept) the FP32 kernel's x-moments pass is branch-free
     // too; its other passes, and the FP64 kernel's, would spill.
     constexpr bool kFlatX8 = K == 3 && !REF && PMAX == 8 && sizeof(Real) == 4;
@@ -1435,7 +1438,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
         }
 #pragma unroll
         for (int k = 0; k < PMAX; ++k)
-            if constexpr (kFlat) {
+            if constexpr (kFlat || kFlatX4) {
                 const bool own = t0 + (int64_t)k * T < half;
                 lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k), xm, xn[2 * k]);
                 lsm_x_keep_flat<Real>(P, own, sval(row, m, 2 * k + 1), xm, xn[2 * k + 1]);
