@@ -855,6 +855,7 @@ struct LsmSync {
                            // by its block count per date
     unsigned int* fin;     // arrivals at the final sums
     unsigned int* err;     // nonzero once any wait gave up
+    unsigned long long* sv;  // self-validating date slots (lsm_publish_sv), zero at rest
 };
 
 constexpr unsigned int kSpinLimit = 1u << 20;
@@ -1029,6 +1030,53 @@ __device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ sl
     }
 }
 
+// Self-validating date slots, the all-reduce of small grids of the quadratic
+// (at most kSvMaxGrid blocks, 1-2 pairs per thread).  A slot word is zero at
+// rest and holds the bitwise complement of
+// a published sum, which is never zero (the complement of an all-ones NaN), so
+// a reader polls the sums themselves: the writer neither waits for its stores
+// nor counts an arrival, and a date costs one trip to the coherence point
+// (stores, then the readers' loads) instead of three (stores and their
+// acknowledgement, the arrival and the counter poll, the loads of the sums).
+// Three parities of slots, sum-major (slot k * grid + block): block b clears
+// its slot of date m+1 once its gather of date m is complete (every block has
+// then read date m+1), and that clear is complete before b publishes date m-1
+// (b waits for its stores first).  A reader of date m-2, the next user of the
+// parity, first saw b's date m-1 sums, so it finds b's slot cleared or
+// rewritten, never holding date m+1.  The kernel's last block clears every
+// parity, so each launch starts from zero.  Word-sized atomic stores and loads
+// (single-copy atomic), as for every other published value.
+// Every waiting lane polls its own block's sums, so the polling grows with
+// the square of the grid: with 24-64 blocks a date took 0.5-2 us less than
+// with arrival counters (same bits), with 256 blocks 0.7-3 us more (the polls
+// crowd the lines being written and the row copies of the next date).
+constexpr int kSvMaxGrid = 64;
+constexpr int kSvParities = 3;
+constexpr int kSvWords = kSvParities * kSvMaxGrid * 8;
+__device__ __forceinline__ unsigned long long* lsm_sv_slots(unsigned long long* sv, int m) {
+    return sv + (size_t)(m % kSvParities) * gridDim.x * 8;
+}
+__device__ __forceinline__ void lsm_publish_sv(double mine, unsigned long long* __restrict__ slots) {
+    if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
+        pcf::stores_complete();      // this wave's clear of date m+2's slot (a date ago) is complete
+        if (threadIdx.x < 8)
+            __hip_atomic_store(slots + (size_t)threadIdx.x * gridDim.x + blockIdx.x,
+                               ~__builtin_bit_cast(unsigned long long, mine), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+// Clear this block's slot of date m+1 (threads 0..7, the publishing lanes),
+// after its gather of date m.
+__device__ __forceinline__ void lsm_clear_sv(unsigned long long* __restrict__ slots) {
+    if (threadIdx.x < 8)
+        __hip_atomic_store(slots + (size_t)threadIdx.x * gridDim.x + blockIdx.x, 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+// A poll (of the slots, or of the arrival counters in lsm_wait_all) re-reads
+// the error word once per kSvErrPoll rounds, not on every round: each read is
+// a trip to the coherence point.
+constexpr unsigned int kSvErrPoll = 64;
+
 __device__ __forceinline__ void lsm_wait_all(const LsmSync& s, unsigned int gen) {
     if (threadIdx.x < pcf::kWave) {  // wave 0, uniformly
         const int lane = threadIdx.x;
@@ -1044,7 +1092,7 @@ __device__ __forceinline__ void lsm_wait_all(const LsmSync& s, unsigned int gen)
             if (__all(v >= target)) break;
             __builtin_amdgcn_s_sleep(1);
             // Past the bound, or once any block has given up, stop waiting.
-            if (++it > kSpinLimit || pcf::fetch_u32(s.err) != 0u) {
+            if (++it > kSpinLimit || (it % kSvErrPoll == 0 && pcf::fetch_u32(s.err) != 0u)) {
                 if (lane == 0) atomicOr(s.err, 1u);
                 break;
             }
@@ -1081,9 +1129,13 @@ constexpr int kLandBlocks = 256;
 // Cache-policy operand of those loads: SC1, the bit pcf::fetch's agent-scope
 // atomic loads carry on gfx950, so both read the published sums the same way.
 constexpr int kLoadCoherent = 16;
-template <bool REF, bool LAND>
-__device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __restrict__ slots, double* red_b,
-                                                 double* tot_sh, double* land) {
+// Gather forms: coherent loads into registers after the arrival wait, loads
+// landed in LDS after that wait, or polled self-validating slots.
+enum { kGatherFetch = 0, kGatherLand = 1, kGatherSv = 2 };
+template <bool REF, int FORM>
+__device__ __forceinline__ void lsm_gather_solve(double cm[4], const void* __restrict__ slots_v, double* red_b,
+                                                 double* tot_sh, double* land, unsigned int* err) {
+    const double* __restrict__ slots = (const double*)slots_v;
     // Only the first min(grid, block) threads fetch; the waves holding them
     // sum in the fixed order of block_sum8 (wave trees, then waves in order),
     // the others stay out.  Thread t adds blocks t, t + blockDim, ... in that
@@ -1095,7 +1147,7 @@ __device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __r
         double t[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) t[k] = 0.0;
-        if constexpr (LAND) {
+        if constexpr (FORM == kGatherLand) {
             for (unsigned int b0 = (unsigned)wave * pcf::kWave; b0 < gridDim.x; b0 += blockDim.x) {
                 if (b0 + lane < gridDim.x) {
                     const double* src = slots + (size_t)(b0 + lane) * 8;
@@ -1117,6 +1169,33 @@ __device__ __forceinline__ void lsm_gather_solve(double cm[4], const double* __r
                         t[2 * j + 1] += row[128 * j + 1];
                     }
                 }
+        } else if constexpr (FORM == kGatherSv) {
+            // A lane whose blocks' sums are not all published yet sums them
+            // again from zero (the same order, so the same bits), until every
+            // lane of the wave has all of them; past the bound, or once any
+            // block has given up, it stops waiting.
+            const unsigned long long* __restrict__ sv = (const unsigned long long*)slots_v;
+            bool miss = true;
+            for (unsigned int it = 1;; ++it) {
+                if (miss) {
+                    miss = false;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) t[k] = 0.0;
+                    for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const unsigned long long u = pcf::fetch_u64(sv + (size_t)k * gridDim.x + bb);
+                            miss |= u == 0ull;
+                            t[k] += __builtin_bit_cast(double, ~u);
+                        }
+                }
+                if (!__any(miss)) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (it > kSpinLimit || (it % kSvErrPoll == 0 && pcf::fetch_u32(err) != 0u)) {
+                    if (miss) atomicOr(err, 1u);
+                    break;
+                }
+            }
         } else {
             for (unsigned int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x)
 #pragma unroll
@@ -1302,8 +1381,10 @@ constexpr int kPersistOut = 8;
 static_assert(kPersistOut <= kOutDoubles, "the persistent results fit the context's pinned result buffer");
 
 // K: regression basis size (3: the quadratic of lsm_solve_g, else the K-term
-// basis of lsm_solve_poly).
-template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
+// basis of lsm_solve_poly).  SV: the quadratic at 1-2 pairs per thread
+// all-reduces through self-validating slots (grids of at most kSvMaxGrid
+// blocks, chosen by the host).
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3, bool SV = false>
 __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs args) {
     using Bs = LsmBasis<K>;
     static_assert(K == 3 || !REF, "the reference-exact mode keeps the reference's basis");
@@ -1317,6 +1398,10 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // thread (lsm_gather_solve).
     constexpr bool kLand = K == 3 && PMAX >= 4;
     __shared__ __attribute__((aligned(16))) double land[kLand ? 8 * kLandBlocks : 2];
+    // Self-validating slots (lsm_publish_sv), with no arrival count.
+    static_assert(!SV || (K == 3 && PMAX <= 2), "self-validating slots: the quadratic at 1-2 pairs per thread");
+    constexpr bool kSv = SV;
+    constexpr int kGather = kLand ? kGatherLand : (kSv ? kGatherSv : kGatherFetch);
     __shared__ double disc[kDiscLds];
     extern __shared__ __align__(16) unsigned char lsm_dyn_lds[];  // row staging (kLdsRow)
     // A private copy: the row copies into LDS count as memory writes, and
@@ -1502,7 +1587,11 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
                     }
                 }
             }
-            lsm_publish<kLand>(block_sum8(acc, red_a), slots_m, sy);
+            const double bs = block_sum8(acc, red_a);
+            if constexpr (kSv)
+                lsm_publish_sv(bs, lsm_sv_slots(sy.sv, m));
+            else
+                lsm_publish<kLand>(bs, slots_m, sy);
         } else {
             double acc[8 * Bs::kChunks];
 #pragma unroll
@@ -1533,10 +1622,15 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
                 x_moments(Sx, m - 1, xm);
             }
         }
-        lsm_wait_all(sy, gen);
+        if constexpr (!kSv) lsm_wait_all(sy, gen);
         if constexpr (K == 3) {
             double cm[4];
-            lsm_gather_solve<REF, kLand>(cm, slots_m, red_b, tot_sh, land);
+            if constexpr (kSv) {
+                lsm_gather_solve<REF, kGather>(cm, lsm_sv_slots(sy.sv, m), red_b, tot_sh, land, sy.err);
+                if (m + 1 <= M - 1) lsm_clear_sv(lsm_sv_slots(sy.sv, m + 1));
+            } else {
+                lsm_gather_solve<REF, kGather>(cm, slots_m, red_b, tot_sh, land, sy.err);
+            }
             if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) b.coef[4 * m + k] = cm[k];
@@ -1634,6 +1728,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
             pcf::publish_u32(sy.fin, 0u);
         }
         if (threadIdx.x < kArrivalCounters) pcf::publish_u32(sy.arrive + threadIdx.x * kCounterStride, 0u);
+        if constexpr (kSv)
+            for (unsigned int i = threadIdx.x; i < kSvParities * gridDim.x * 8; i += blockDim.x)
+                __hip_atomic_store(sy.sv + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1644,7 +1741,7 @@ size_t lsm_dyn_lds_bytes(int threads) {
     return lsm_lds_row_bytes<Real, PMAX>(threads);
 }
 
-template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
+template <typename Real, int SRC, bool REF, int PMAX, int K = 3, bool SV = false>
 hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStream_t s, hipEvent_t ev0,
                                hipEvent_t ev1) {
     LsmPersistArgs a = pa;
@@ -1656,8 +1753,20 @@ hipError_t launch_persistent_t(const LsmPersistArgs& pa, int grid, int B, hipStr
     // become resident -- at worst after a co-running kernel ends.  Every wait
     // is bounded all the same.  ev0 / ev1 take the kernel's
     // own start and end times (no marker packets between consecutive launches).
-    return hipExtLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K>, dim3((unsigned)grid),
+    return hipExtLaunchKernel((const void*)k_lsm_persistent<Real, SRC, REF, PMAX, K, SV>, dim3((unsigned)grid),
                               dim3((unsigned)B), args, lsm_dyn_lds_bytes<Real, SRC, REF, PMAX>(B), s, ev0, ev1, 0);
+}
+
+// The quadratic at 1-2 pairs per thread: self-validating slots for grids of
+// at most kSvMaxGrid blocks, arrival counters above.  (Co-residency and LDS
+// are checked on the counted kernel: the two have the same LDS, and each holds
+// one 1024-thread block per CU at most.)
+template <typename Real, int SRC, bool REF, int PMAX>
+hipError_t launch_persistent_q(const LsmPersistArgs& pa, int grid, int B, hipStream_t s, hipEvent_t ev0,
+                               hipEvent_t ev1) {
+    if constexpr (PMAX <= 2)
+        if (grid <= kSvMaxGrid) return launch_persistent_t<Real, SRC, REF, PMAX, 3, true>(pa, grid, B, s, ev0, ev1);
+    return launch_persistent_t<Real, SRC, REF, PMAX>(pa, grid, B, s, ev0, ev1);
 }
 
 template <typename Real, int SRC, bool REF, int PMAX, int K = 3>
@@ -1696,14 +1805,15 @@ bool lds_fits_t(int B, int device) {
 template <int PMAX, int K>
 hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid, int B,
                                hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(pa, grid, B, s, e0, e1);
     if constexpr (K == 3) {
+        if (!f64) return launch_persistent_q<float, kPhilox, false, PMAX>(pa, grid, B, s, e0, e1);
         if (src == kHost)
-            return ref ? launch_persistent_t<double, kHost, true, PMAX>(pa, grid, B, s, e0, e1)
-                       : launch_persistent_t<double, kHost, false, PMAX>(pa, grid, B, s, e0, e1);
-        return ref ? launch_persistent_t<double, kPhilox, true, PMAX>(pa, grid, B, s, e0, e1)
-                   : launch_persistent_t<double, kPhilox, false, PMAX>(pa, grid, B, s, e0, e1);
+            return ref ? launch_persistent_q<double, kHost, true, PMAX>(pa, grid, B, s, e0, e1)
+                       : launch_persistent_q<double, kHost, false, PMAX>(pa, grid, B, s, e0, e1);
+        return ref ? launch_persistent_q<double, kPhilox, true, PMAX>(pa, grid, B, s, e0, e1)
+                   : launch_persistent_q<double, kPhilox, false, PMAX>(pa, grid, B, s, e0, e1);
     } else {
+        if (!f64) return launch_persistent_t<float, kPhilox, false, PMAX, K>(pa, grid, B, s, e0, e1);
         (void)ref;
         return src == kHost ? launch_persistent_t<double, kHost, false, PMAX, K>(pa, grid, B, s, e0, e1)
                             : launch_persistent_t<double, kPhilox, false, PMAX, K>(pa, grid, B, s, e0, e1);
@@ -1963,9 +2073,9 @@ PcfAmerBytes pcf_amer_bytes(int64_t N, int M, int precision, int rng, int ref_ex
     PcfAmerBytes b;
     b.store = (int64_t)M * sh.cap * rs;
     b.state = sh.cap * (4 + 2 * rs);
-    // coefficients, discounts, parameters and sync words + the persistent
+    // coefficients, discounts, parameters, sync words and slots + the persistent
     // form's partials: at most 4096 co-resident blocks x 448 B (6-term basis)
-    b.rest = grid_max * 10 * 16 + 8 * (9 * (int64_t)(M + 1)) + 8192 + 4096 * 448;
+    b.rest = grid_max * 10 * 16 + 8 * (9 * (int64_t)(M + 1)) + 8192 + 8 * kSvWords + 4096 * 448;
     return b;
 }
 
@@ -2057,10 +2167,11 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
     const LsmShape sh = lsm_shape(N, mode, o, ctx->cu_count);
     const size_t store_bytes = (size_t)M * (size_t)sh.cap * rs;
     const size_t state_bytes = (size_t)sh.cap * (sizeof(int32_t) + 2 * rs);
-    // persistent-form sync words (at a fixed place, zero at rest: zeroed when
-    // allocated, and every persistent launch leaves them zero) | coef (CS per
-    // date) | discount table | parameter block
-    const size_t sync_bytes = (sizeof(unsigned int) * kSyncWords + 255) / 256 * 256;
+    // persistent-form sync words and self-validating slots (at a fixed place,
+    // zero at rest: zeroed when allocated, and every persistent launch leaves
+    // them zero) | coef (CS per date) | discount table | parameter block
+    const size_t words_bytes = (sizeof(unsigned int) * kSyncWords + 255) / 256 * 256;
+    const size_t sync_bytes = words_bytes + sizeof(unsigned long long) * kSvWords;
     const size_t coef_bytes =
         sync_bytes + sizeof(double) * (CS * (size_t)(M + 1) + (size_t)(M + 1)) + sizeof(LsmParams);
     PCF_HIP_CHECK(ctx, ctx->lsm_store.ensure(store_bytes));
@@ -2128,7 +2239,8 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
         else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
         if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
-        // The quadratic form gathers at most kLandBlocks blocks' sums through LDS.
+        // The quadratic form gathers at most kLandBlocks blocks' sums (through
+        // LDS, or from the self-validating slots).
         if (K == 3 && gmax > kLandBlocks) gmax = kLandBlocks;
         if (g >= 1 && g <= gmax) {
             persistent = true;
@@ -2192,7 +2304,8 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         const int fin_w = kArrivalCounters * kCounterStride;
         LsmPersistArgs pa;
         pa.b = b;
-        pa.sy = LsmSync{sync_words, sync_words + fin_w, sync_words + fin_w + 1};
+        pa.sy = LsmSync{sync_words, sync_words + fin_w, sync_words + fin_w + 1,
+                        (unsigned long long*)(ctx->lsm_coef.as<char>() + words_bytes)};
         pa.P = P;
         for (int k = 0; k <= M && k < kDiscArgs; ++k) pa.disc[k] = host_blk[CS * (size_t)(M + 1) + k];
         hipStream_t cs = nullptr;

@@ -225,6 +225,9 @@ __device__ __forceinline__ void publish_u32(unsigned int* p, unsigned int v) {
 __device__ __forceinline__ unsigned int fetch_u32(const unsigned int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned long long fetch_u64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // Wait until every memory operation this wave issued is complete.
 __device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
 

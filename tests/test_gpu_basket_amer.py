@@ -395,6 +395,32 @@ def test_amer_persistent_matches_graph_form(engine, N, M, prec, ref):
     assert g.control_price == p.control_price or abs(g.control_price - p.control_price) < 1e-12 * g.control_price
 
 
+def test_amer_small_grid_slots_across_launches(engine):
+    """Grids of at most 64 blocks all-reduce through self-validating slots
+    that every launch leaves zero: a 64-block sweep over 199 dates (each slot
+    parity reused 66 times) matches the per-date graph, and repeats bit for
+    bit after launches of other grid sizes (32 blocks, 256 blocks through the
+    arrival counters) and another date count on the same context."""
+    kw = dict(seed=37, return_when=True)
+    first = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 14, 200, "put", kernel_variant=2, precision="fp32k", **kw)
+    say("  small grid", first.grid_blocks, "x", first.block_size, "kernel_ms", first.kernel_ms)
+    assert first.grid_blocks <= 64
+    g = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 14, 200, "put", kernel_variant=1, precision="fp32k", **kw)
+    flips = int((g.extra["when"] != first.extra["when"]).sum())
+    assert flips <= 2
+    if flips == 0:
+        assert abs(g.price - first.price) <= 1e-12 * g.price
+    else:
+        assert abs(g.price - first.price) <= 0.01 * g.std_err
+    others = [engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 13, 37, "put", kernel_variant=2, precision="fp64", **kw),
+              engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 18, 50, "put", kernel_variant=2, precision="fp32k", **kw)]
+    say("  between:", [(o.grid_blocks, o.block_size) for o in others])
+    assert others[0].grid_blocks <= 64 < others[1].grid_blocks
+    again = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 14, 200, "put", kernel_variant=2, precision="fp32k", **kw)
+    assert (again.price, again.std_err) == (first.price, first.std_err)
+    assert np.array_equal(again.extra["when"], first.extra["when"])
+
+
 def test_amer_quadratic_persistent_grid_capped_at_256_blocks(engine):
     """The quadratic's persistent form gathers at most 256 blocks' sums per
     date through LDS: a forced shape of 512 blocks (64 threads x 1 pair at
