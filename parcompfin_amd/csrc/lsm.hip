@@ -1969,14 +1969,18 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 
 // Default shape of the persistent form.  Per date every block reads every
 // block's partials (grid^2 coherent loads) and each CU works through its own
-// pairs with latency exposed at few waves, so the grid targets 64 blocks for
-// up to 2^13 antithetic pairs, 128 up to 2^15 and one per CU beyond, and the
-// pairs of a block are spread over as many threads as a block holds (1024)
-// before a thread takes more than one pair: the fewest pairs per thread (1..8)
-// that fit, not only powers of two (the kernel's register budget is the next
-// power of two; the pair slots past the count hold no path).
+// pairs with latency exposed at few waves, so the grid targets 64 blocks (the
+// self-validating all-reduce) for up to 2^15 antithetic pairs and one per CU
+// beyond, and the pairs of a block are spread over as many threads as a block
+// holds (1024) before a thread takes more than one pair: the fewest pairs per
+// thread (1..8) that fit, not only powers of two (the kernel's register budget
+// is the next power of two; the pair slots past the count hold no path).
+// (At 2^15 and 2^16 paths, 64 blocks of 256 and 512 threads took 0.121 and
+// 0.138 ms in FP32, 0.149 and 0.171 ms in FP64, against 0.161, 0.167, 0.182
+// and 0.190 ms for the 128 counted blocks before; at 2^17 paths FP64 was
+// faster with 256 blocks, 0.212 against 0.227 ms.)
 void lsm_persistent_shape(int64_t half, int cus, int* bs, int* pm) {
-    const int64_t target = half <= (1 << 13) ? 64 : half <= (1 << 15) ? 128 : std::max(cus, 64);
+    const int64_t target = half <= (1 << 15) ? kSvMaxGrid : std::max(cus, 64);
     const int64_t per_block = (half + target - 1) / target;
     int p = 1;
     while (p < 8 && (per_block + p - 1) / p > 1024) ++p;
