@@ -1970,19 +1970,25 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 // Default shape of the persistent form.  Per date every block reads every
 // block's partials (grid^2 coherent loads) and each CU works through its own
 // pairs with latency exposed at few waves, so the grid targets 64 blocks (the
-// self-validating all-reduce) for up to 2^15 antithetic pairs and one per CU
-// beyond, and the pairs of a block are spread over as many threads as a block
-// holds (1024) before a thread takes more than one pair: the fewest pairs per
-// thread (1..8) that fit, not only powers of two (the kernel's register budget
-// is the next power of two; the pair slots past the count hold no path).
+// self-validating all-reduce) for up to 2^15 antithetic pairs in FP64 and 2^17
+// in FP32, and one per CU beyond, and the pairs of a block are spread over as
+// many threads as a block holds (1024) before a thread takes more than one
+// pair (FP32 above 2^15 pairs: 2 pairs per thread from the start): the fewest
+// pairs per thread (1..8) that fit, not only powers of two (the kernel's
+// register budget is the next power of two; the pair slots past the count
+// hold no path).
 // (At 2^15 and 2^16 paths, 64 blocks of 256 and 512 threads took 0.121 and
 // 0.138 ms in FP32, 0.149 and 0.171 ms in FP64, against 0.161, 0.167, 0.182
-// and 0.190 ms for the 128 counted blocks before; at 2^17 paths FP64 was
-// faster with 256 blocks, 0.212 against 0.227 ms.)
-void lsm_persistent_shape(int64_t half, int cus, int* bs, int* pm) {
-    const int64_t target = half <= (1 << 15) ? kSvMaxGrid : std::max(cus, 64);
+// and 0.190 ms for the 128 counted blocks before.)  In FP32 the 64 blocks
+// at 2 pairs per thread also took 76500-178714 paths 10-18% faster than the
+// 200-256 counted blocks (2^17 paths 0.158 against 0.182 ms, and 0.167 at
+// 1024 threads x 1 pair; 163840 paths 0.173 against 0.192) and 250052 paths
+// as fast; in FP64 256 blocks stay ahead at 2^17 paths (0.212 against 0.227).
+void lsm_persistent_shape(int64_t half, bool f64, int cus, int* bs, int* pm) {
+    const int64_t sv_pairs = f64 ? (1 << 15) : (1 << 17);
+    const int64_t target = half <= sv_pairs ? kSvMaxGrid : std::max(cus, 64);
     const int64_t per_block = (half + target - 1) / target;
-    int p = 1;
+    int p = !f64 && half > (1 << 15) && half <= sv_pairs ? 2 : 1;
     while (p < 8 && (per_block + p - 1) / p > 1024) ++p;
     const int64_t b = ((per_block + p - 1) / p + 63) / 64 * 64;
     *pm = p;
@@ -2217,12 +2223,12 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         int bs = 0, pm = 0;
         const bool forced = o.block_size > 0 || o.items_per_thread > 0;
         if (forced) {
-            lsm_persistent_shape(P.half, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
+            lsm_persistent_shape(P.half, f64, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
             if (o.block_size > 0) bs = o.block_size;
             if (o.items_per_thread > 0) pm = o.items_per_thread;
         } else {
             if (K == 3) pcf_tuned_shape(PCF_KIND_AMER, f64 ? 1 : 0, N, &bs, &pm);  // tuned for the quadratic
-            if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
+            if (bs <= 0 || pm <= 0) lsm_persistent_shape(P.half, f64, std::min(ctx->cu_count, kLandBlocks), &bs, &pm);
         }
         if (bs % 64 || bs < 64 || bs > 1024 || pm < 1 || pm > 8)
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,

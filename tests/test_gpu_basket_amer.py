@@ -413,7 +413,7 @@ def test_amer_small_grid_slots_across_launches(engine):
     else:
         assert abs(g.price - first.price) <= 0.01 * g.std_err
     others = [engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 13, 37, "put", kernel_variant=2, precision="fp64", **kw),
-              engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 18, 50, "put", kernel_variant=2, precision="fp32k", **kw)]
+              engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 19, 50, "put", kernel_variant=2, precision="fp32k", **kw)]
     say("  between:", [(o.grid_blocks, o.block_size) for o in others])
     assert others[0].grid_blocks <= 64 < others[1].grid_blocks
     again = engine.mc_amer(100, 100, 0.06, 0.2, 1.0, 1 << 14, 200, "put", kernel_variant=2, precision="fp32k", **kw)
