@@ -1117,7 +1117,7 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // FP64 chain (reciprocals) in turn.  The solve is a pure
 // function of the totals, which are the same bits in every block, so every
 // block still derives the same coefficients.
-// With LAND (kernels of 4 or 8 pairs per thread), each lane's block published
+// With LAND (kernels of 4, 6 or 8 pairs per thread), each lane's block published
 // 64 contiguous bytes, and four 16-byte coherent loads per lane land them in
 // LDS (global_load_lds: no registers), all in flight at once.  Through
 // registers, these kernels sit at the 128-VGPR limit and the compiler issued
@@ -1394,7 +1394,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
     __shared__ double red_a[16 * 8 * Bs::kChunks], red_b[16 * 8 * Bs::kChunks], tot_sh[8 * Bs::kChunks];
-    // The per-date gather lands in LDS for the quadratic at 4 or 8 pairs per
+    // The per-date gather lands in LDS for the quadratic at 4, 6 or 8 pairs per
     // thread (lsm_gather_solve).
     constexpr bool kLand = K == 3 && PMAX >= 4;
     __shared__ __attribute__((aligned(16))) double land[kLand ? 8 * kLandBlocks : 2];
@@ -1481,7 +1481,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // come from the LDS row staging (kLdsRow: the row of date m-1 is copied in
     // at the top of date m) or, where two staged rows do not fit, from
     // registers: Sc holds date m and date m-1 is loaded into Sn after the
-    // block's arrival (at 8 pairs per thread the row of m-1 is loaded into
+    // block's arrival (at 6-8 pairs per thread the row of m-1 is loaded into
     // transient registers instead, Sc being reloaded at the next date).
     constexpr bool kLdsRow = lsm_lds_row<Real, PMAX>();
     constexpr bool kPrefetch = !kLdsRow && PMAX <= 4;
@@ -1511,9 +1511,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // At 4 pairs only the x-moments pass is branch-free: the other two would
     // spill.
     constexpr bool kFlatX4 = kKeepX && PMAX == 4;
-    // At 8 pairs (x not kept) the FP32 kernel's x-moments pass is branch-free
-    // too; its other passes, and the FP64 kernel's, would spill.
-    constexpr bool kFlatX8 = K == 3 && !REF && PMAX == 8 && sizeof(Real) == 4;
+    // At 6 and 8 pairs (x not kept) the FP32 kernel's x-moments pass is
+    // branch-free too; its other passes, and the FP64 kernel's, would spill.
+    constexpr bool kFlatX8 = K == 3 && !REF && PMAX >= 6 && sizeof(Real) == 4;
     auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
         for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
@@ -1822,6 +1822,7 @@ hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs
 template <int PMAX>
 hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid,
                              int B, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if constexpr (PMAX == 6) return launch_persistent_k<6, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
     switch (K) {
         case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, e0, e1);
         case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, e0, e1);
@@ -1848,6 +1849,7 @@ bool lds_fits_k(bool f64, int src, bool ref, int B, int dev) {
 }
 template <int PMAX>
 bool lds_fits(int K, bool f64, int src, bool ref, int B, int dev) {
+    if constexpr (PMAX == 6) return lds_fits_k<6, 3>(f64, src, ref, B, dev);
     switch (K) {
         case 2: return lds_fits_k<PMAX, 2>(f64, src, ref, B, dev);
         case 4: return lds_fits_k<PMAX, 4>(f64, src, ref, B, dev);
@@ -1874,6 +1876,7 @@ int max_coresident_k(bool f64, int src, bool ref, int B, int cus, int dev) {
 }
 template <int PMAX>
 int max_coresident(int K, bool f64, int src, bool ref, int B, int cus, int dev) {
+    if constexpr (PMAX == 6) return max_coresident_k<6, 3>(f64, src, ref, B, cus, dev);
     switch (K) {
         case 2: return max_coresident_k<PMAX, 2>(f64, src, ref, B, cus, dev);
         case 4: return max_coresident_k<PMAX, 4>(f64, src, ref, B, cus, dev);
@@ -2234,19 +2237,24 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                             "persistent LSM shape %d x %d: need a block size in 64..1024 (multiple of 64) and "
                             "1..8 pairs per thread", bs, pm);
-        int p2 = 1;  // register budget: pair slots per thread, a power of two
+        // Register budget: pair slots per thread, a power of two, or 6 for 5-6
+        // pairs of the quadratic.
+        int p2 = 1;
         while (p2 < pm) p2 <<= 1;
+        if (K == 3 && (pm == 5 || pm == 6)) p2 = 6;
         // pm pairs per thread: slots pm..p2-1 hold no path anywhere
         const int64_t g = (P.half + (int64_t)pm * bs - 1) / ((int64_t)pm * bs);
         int gmax;
         if (p2 == 1) gmax = max_coresident<1>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 2) gmax = max_coresident<2>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 4) gmax = max_coresident<4>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 6) gmax = max_coresident<6>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else gmax = max_coresident<8>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         bool lds_ok;
         if (p2 == 1) lds_ok = lds_fits<1>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 2) lds_ok = lds_fits<2>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
+        else if (p2 == 6) lds_ok = lds_fits<6>(K, f64, src, ref, bs, ctx->device);
         else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
         if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
         // The quadratic form gathers at most kLandBlocks blocks' sums (through
@@ -2336,6 +2344,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else if (pmax == 6) le = launch_persistent<6>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
             PersistEnd& pe = persist_end()[ctx->device];
