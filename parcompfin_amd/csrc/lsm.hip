@@ -1978,7 +1978,7 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 // many threads as a block holds (1024) before a thread takes more than one
 // pair (FP32 above 2^15 pairs: 2 pairs per thread from the start): the fewest
 // pairs per thread (1..8) that fit, not only powers of two (the kernel's
-// register budget is the next power of two; the pair slots past the count
+// register budget is the next power of two, or 6; the pair slots past the count
 // hold no path).
 // (At 2^15 and 2^16 paths, 64 blocks of 256 and 512 threads took 0.121 and
 // 0.138 ms in FP32, 0.149 and 0.171 ms in FP64, against 0.161, 0.167, 0.182
