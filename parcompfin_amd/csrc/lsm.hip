@@ -1117,7 +1117,7 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // FP64 chain (reciprocals) in turn.  The solve is a pure
 // function of the totals, which are the same bits in every block, so every
 // block still derives the same coefficients.
-// With LAND (kernels of 4, 6 or 8 pairs per thread), each lane's block published
+// With LAND (kernels of 3, 4, 6 or 8 pairs per thread), each lane's block published
 // 64 contiguous bytes, and four 16-byte coherent loads per lane land them in
 // LDS (global_load_lds: no registers), all in flight at once.  Through
 // registers, these kernels sit at the 128-VGPR limit and the compiler issued
@@ -1394,9 +1394,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     __shared__ pcf::CSum lds[16 * 8];
     __shared__ double ldsp[16 * 8];
     __shared__ double red_a[16 * 8 * Bs::kChunks], red_b[16 * 8 * Bs::kChunks], tot_sh[8 * Bs::kChunks];
-    // The per-date gather lands in LDS for the quadratic at 4, 6 or 8 pairs per
+    // The per-date gather lands in LDS for the quadratic at 3-8 pairs per
     // thread (lsm_gather_solve).
-    constexpr bool kLand = K == 3 && PMAX >= 4;
+    constexpr bool kLand = K == 3 && PMAX >= 3;
     __shared__ __attribute__((aligned(16))) double land[kLand ? 8 * kLandBlocks : 2];
     // Self-validating slots (lsm_publish_sv), with no arrival count.
     static_assert(!SV || (K == 3 && PMAX <= 2), "self-validating slots: the quadratic at 1-2 pairs per thread");
@@ -1508,9 +1508,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     constexpr bool kKeepX = K == 3 && !REF && PMAX <= 4;
     double xc[kKeepX ? 2 * PMAX : 1], xn[kKeepX ? 2 * PMAX : 1];
     constexpr bool kFlat = kKeepX && PMAX <= 2;  // branch-free passes (lsm_x_keep_flat)
-    // At 4 pairs only the x-moments pass is branch-free: the other two would
-    // spill.
-    constexpr bool kFlatX4 = kKeepX && PMAX == 4;
+    // At 3 and 4 pairs only the x-moments pass is branch-free: the other two
+    // would spill.
+    constexpr bool kFlatX4 = kKeepX && PMAX >= 3;
     // At 6 and 8 pairs (x not kept) the FP32 kernel's x-moments pass is
     // branch-free too; its other passes, and the FP64 kernel's, would spill.
     constexpr bool kFlatX8 = K == 3 && !REF && PMAX >= 6 && sizeof(Real) == 4;
@@ -1822,7 +1822,7 @@ hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs
 template <int PMAX>
 hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid,
                              int B, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if constexpr (PMAX == 6) return launch_persistent_k<6, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
+    if constexpr (PMAX == 3 || PMAX == 6) return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
     switch (K) {
         case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, e0, e1);
         case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, e0, e1);
@@ -1849,7 +1849,7 @@ bool lds_fits_k(bool f64, int src, bool ref, int B, int dev) {
 }
 template <int PMAX>
 bool lds_fits(int K, bool f64, int src, bool ref, int B, int dev) {
-    if constexpr (PMAX == 6) return lds_fits_k<6, 3>(f64, src, ref, B, dev);
+    if constexpr (PMAX == 3 || PMAX == 6) return lds_fits_k<PMAX, 3>(f64, src, ref, B, dev);
     switch (K) {
         case 2: return lds_fits_k<PMAX, 2>(f64, src, ref, B, dev);
         case 4: return lds_fits_k<PMAX, 4>(f64, src, ref, B, dev);
@@ -1876,7 +1876,7 @@ int max_coresident_k(bool f64, int src, bool ref, int B, int cus, int dev) {
 }
 template <int PMAX>
 int max_coresident(int K, bool f64, int src, bool ref, int B, int cus, int dev) {
-    if constexpr (PMAX == 6) return max_coresident_k<6, 3>(f64, src, ref, B, cus, dev);
+    if constexpr (PMAX == 3 || PMAX == 6) return max_coresident_k<PMAX, 3>(f64, src, ref, B, cus, dev);
     switch (K) {
         case 2: return max_coresident_k<PMAX, 2>(f64, src, ref, B, cus, dev);
         case 4: return max_coresident_k<PMAX, 4>(f64, src, ref, B, cus, dev);
@@ -2241,18 +2241,20 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         // pairs of the quadratic.
         int p2 = 1;
         while (p2 < pm) p2 <<= 1;
-        if (K == 3 && (pm == 5 || pm == 6)) p2 = 6;
+        if (K == 3 && (pm == 3 || pm == 5 || pm == 6)) p2 = pm == 3 ? 3 : 6;
         // pm pairs per thread: slots pm..p2-1 hold no path anywhere
         const int64_t g = (P.half + (int64_t)pm * bs - 1) / ((int64_t)pm * bs);
         int gmax;
         if (p2 == 1) gmax = max_coresident<1>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 2) gmax = max_coresident<2>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 3) gmax = max_coresident<3>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 4) gmax = max_coresident<4>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 6) gmax = max_coresident<6>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else gmax = max_coresident<8>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         bool lds_ok;
         if (p2 == 1) lds_ok = lds_fits<1>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 2) lds_ok = lds_fits<2>(K, f64, src, ref, bs, ctx->device);
+        else if (p2 == 3) lds_ok = lds_fits<3>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 6) lds_ok = lds_fits<6>(K, f64, src, ref, bs, ctx->device);
         else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
@@ -2343,6 +2345,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             hipError_t le;
             if (pmax == 1) le = launch_persistent<1>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else if (pmax == 3) le = launch_persistent<3>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 6) le = launch_persistent<6>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
