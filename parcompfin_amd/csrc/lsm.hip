@@ -1481,7 +1481,7 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // come from the LDS row staging (kLdsRow: the row of date m-1 is copied in
     // at the top of date m) or, where two staged rows do not fit, from
     // registers: Sc holds date m and date m-1 is loaded into Sn after the
-    // block's arrival (at 6-8 pairs per thread the row of m-1 is loaded into
+    // block's arrival (at 5-8 pairs per thread the row of m-1 is loaded into
     // transient registers instead, Sc being reloaded at the next date).
     constexpr bool kLdsRow = lsm_lds_row<Real, PMAX>();
     constexpr bool kPrefetch = !kLdsRow && PMAX <= 4;
@@ -1511,9 +1511,9 @@ __global__ __launch_bounds__(1024) void k_lsm_persistent(const LsmPersistArgs ar
     // At 3 and 4 pairs only the x-moments pass is branch-free: the other two
     // would spill.
     constexpr bool kFlatX4 = kKeepX && PMAX >= 3;
-    // At 6 and 8 pairs (x not kept) the FP32 kernel's x-moments pass is
+    // At 5-8 pairs (x not kept) the FP32 kernel's x-moments pass is
     // branch-free too; its other passes, and the FP64 kernel's, would spill.
-    constexpr bool kFlatX8 = K == 3 && !REF && PMAX >= 6 && sizeof(Real) == 4;
+    constexpr bool kFlatX8 = K == 3 && !REF && PMAX >= 5 && sizeof(Real) == 4;
     auto x_moments = [&](const Real* row, int m, double xm[Bs::kMom]) {
 #pragma unroll
         for (int k = 0; k < Bs::kMom; ++k) xm[k] = 0.0;
@@ -1822,7 +1822,7 @@ hipError_t launch_persistent_k(bool f64, int src, bool ref, const LsmPersistArgs
 template <int PMAX>
 hipError_t launch_persistent(int K, bool f64, int src, bool ref, const LsmPersistArgs& pa, int grid,
                              int B, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if constexpr (PMAX == 3 || PMAX == 6) return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
+    if constexpr (PMAX == 3 || PMAX == 5 || PMAX == 6) return launch_persistent_k<PMAX, 3>(f64, src, ref, pa, grid, B, s, e0, e1);
     switch (K) {
         case 2: return launch_persistent_k<PMAX, 2>(f64, src, ref, pa, grid, B, s, e0, e1);
         case 4: return launch_persistent_k<PMAX, 4>(f64, src, ref, pa, grid, B, s, e0, e1);
@@ -1849,7 +1849,7 @@ bool lds_fits_k(bool f64, int src, bool ref, int B, int dev) {
 }
 template <int PMAX>
 bool lds_fits(int K, bool f64, int src, bool ref, int B, int dev) {
-    if constexpr (PMAX == 3 || PMAX == 6) return lds_fits_k<PMAX, 3>(f64, src, ref, B, dev);
+    if constexpr (PMAX == 3 || PMAX == 5 || PMAX == 6) return lds_fits_k<PMAX, 3>(f64, src, ref, B, dev);
     switch (K) {
         case 2: return lds_fits_k<PMAX, 2>(f64, src, ref, B, dev);
         case 4: return lds_fits_k<PMAX, 4>(f64, src, ref, B, dev);
@@ -1876,7 +1876,7 @@ int max_coresident_k(bool f64, int src, bool ref, int B, int cus, int dev) {
 }
 template <int PMAX>
 int max_coresident(int K, bool f64, int src, bool ref, int B, int cus, int dev) {
-    if constexpr (PMAX == 3 || PMAX == 6) return max_coresident_k<PMAX, 3>(f64, src, ref, B, cus, dev);
+    if constexpr (PMAX == 3 || PMAX == 5 || PMAX == 6) return max_coresident_k<PMAX, 3>(f64, src, ref, B, cus, dev);
     switch (K) {
         case 2: return max_coresident_k<PMAX, 2>(f64, src, ref, B, cus, dev);
         case 4: return max_coresident_k<PMAX, 4>(f64, src, ref, B, cus, dev);
@@ -1978,7 +1978,7 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 // many threads as a block holds (1024) before a thread takes more than one
 // pair (FP32 above 2^15 pairs: 2 pairs per thread from the start): the fewest
 // pairs per thread (1..8) that fit, not only powers of two (the kernel's
-// register budget is the next power of two, or 6; the pair slots past the count
+// register budget is the next power of two, or 3, 5 or 6; the pair slots past the count
 // hold no path).
 // (At 2^15 and 2^16 paths, 64 blocks of 256 and 512 threads took 0.121 and
 // 0.138 ms in FP32, 0.149 and 0.171 ms in FP64, against 0.161, 0.167, 0.182
@@ -2237,11 +2237,11 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             return pcf_fail(ctx, PCF_ERR_INVALID_ARG,
                             "persistent LSM shape %d x %d: need a block size in 64..1024 (multiple of 64) and "
                             "1..8 pairs per thread", bs, pm);
-        // Register budget: pair slots per thread, a power of two, or 6 for 5-6
-        // pairs of the quadratic.
+        // Register budget: pair slots per thread, a power of two, or exactly
+        // 3, 5 or 6 pairs of the quadratic.
         int p2 = 1;
         while (p2 < pm) p2 <<= 1;
-        if (K == 3 && (pm == 3 || pm == 5 || pm == 6)) p2 = pm == 3 ? 3 : 6;
+        if (K == 3 && (pm == 3 || pm == 5 || pm == 6)) p2 = pm;
         // pm pairs per thread: slots pm..p2-1 hold no path anywhere
         const int64_t g = (P.half + (int64_t)pm * bs - 1) / ((int64_t)pm * bs);
         int gmax;
@@ -2249,6 +2249,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         else if (p2 == 2) gmax = max_coresident<2>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 3) gmax = max_coresident<3>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 4) gmax = max_coresident<4>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
+        else if (p2 == 5) gmax = max_coresident<5>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else if (p2 == 6) gmax = max_coresident<6>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         else gmax = max_coresident<8>(K, f64, src, ref, bs, ctx->cu_count, ctx->device);
         bool lds_ok;
@@ -2256,6 +2257,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
         else if (p2 == 2) lds_ok = lds_fits<2>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 3) lds_ok = lds_fits<3>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 4) lds_ok = lds_fits<4>(K, f64, src, ref, bs, ctx->device);
+        else if (p2 == 5) lds_ok = lds_fits<5>(K, f64, src, ref, bs, ctx->device);
         else if (p2 == 6) lds_ok = lds_fits<6>(K, f64, src, ref, bs, ctx->device);
         else lds_ok = lds_fits<8>(K, f64, src, ref, bs, ctx->device);
         if (!lds_ok) gmax = 0;  // the row staging would not fit: graph form
@@ -2347,6 +2349,7 @@ int mc_amer_run(pcf_ctx* ctx, double S0, double E, double r, double sigma, doubl
             else if (pmax == 2) le = launch_persistent<2>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 3) le = launch_persistent<3>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 4) le = launch_persistent<4>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
+            else if (pmax == 5) le = launch_persistent<5>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else if (pmax == 6) le = launch_persistent<6>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             else le = launch_persistent<8>(K, f64, src, ref, pa, pgrid, PB, cs, e0, e1);
             if (le != hipSuccess) return pcf_hip_fail(ctx, le, "launching the persistent LSM kernel");
