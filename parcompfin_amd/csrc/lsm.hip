@@ -1032,9 +1032,9 @@ __device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ sl
 
 // Self-validating date slots, the all-reduce of small grids of the quadratic
 // (at most kSvMaxGrid blocks, 1-2 pairs per thread).  A slot word is zero at
-// rest and holds the bitwise complement of
-// a published sum, which is never zero (the complement of an all-ones NaN), so
-// a reader polls the sums themselves: the writer neither waits for its stores
+// rest and holds the bitwise complement of a published sum, which is never
+// zero (the complement of an all-ones NaN), so a reader polls the sums
+// themselves: the writer neither waits for its stores
 // nor counts an arrival, and a date costs one trip to the coherence point
 // (stores, then the readers' loads) instead of three (stores and their
 // acknowledgement, the arrival and the counter poll, the loads of the sums).
@@ -1978,11 +1978,11 @@ LsmShape lsm_shape(int64_t N, int mode, const pcf_opts& o, int cus = 256) {
 // many threads as a block holds (1024) before a thread takes more than one
 // pair (FP32 above 2^15 pairs: 2 pairs per thread from the start): the fewest
 // pairs per thread (1..8) that fit, not only powers of two (the kernel's
-// register budget is the next power of two, or 3, 5 or 6; the pair slots past the count
-// hold no path).
+// register budget is the next power of two, or exactly 3, 5 or 6 pairs; the
+// pair slots past the count hold no path).
 // (At 2^15 and 2^16 paths, 64 blocks of 256 and 512 threads took 0.121 and
 // 0.138 ms in FP32, 0.149 and 0.171 ms in FP64, against 0.161, 0.167, 0.182
-// and 0.190 ms for the 128 counted blocks before.)  In FP32 the 64 blocks
+// and 0.190 ms with 128 blocks and arrival counters.)  In FP32 the 64 blocks
 // at 2 pairs per thread also took 76500-178714 paths 10-18% faster than the
 // 200-256 counted blocks (2^17 paths 0.158 against 0.182 ms, and 0.167 at
 // 1024 threads x 1 pair; 163840 paths 0.173 against 0.192) and 250052 paths
