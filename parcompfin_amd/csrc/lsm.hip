@@ -1034,10 +1034,10 @@ __device__ __forceinline__ void lsm_publish(double mine, double* __restrict__ sl
 // (at most kSvMaxGrid blocks, 1-2 pairs per thread).  A slot word is zero at
 // rest and holds the bitwise complement of a published sum, which is never
 // zero (the complement of an all-ones NaN), so a reader polls the sums
-// themselves: the writer neither waits for its stores
-// nor counts an arrival, and a date costs one trip to the coherence point
-// (stores, then the readers' loads) instead of three (stores and their
-// acknowledgement, the arrival and the counter poll, the loads of the sums).
+// themselves: the writer neither waits for its stores nor counts an arrival,
+// and a date costs one trip to the coherence point (stores, then the readers'
+// loads) instead of three (stores and their acknowledgement, the arrival and
+// the counter poll, the loads of the sums).
 // Three parities of slots, sum-major (slot k * grid + block): block b clears
 // its slot of date m+1 once its gather of date m is complete (every block has
 // then read date m+1), and that clear is complete before b publishes date m-1
